@@ -1,0 +1,82 @@
+/*
+ * wxo - CPU restatement (plain C, FP64) of Walicxe3D's per-block Godunov
+ * update, used ONLY as the correctness oracle of the GPU library.
+ *
+ * TEST INFRASTRUCTURE.  Only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load or call this library; the product path
+ * (walicxe3d_amd) never does.
+ *
+ * State layout: per block, per variable, a 20^3 cube with two ghost layers,
+ * indices i,j,k in [-1,18] as in the reference (source/parameters.f90:356-361),
+ * i fastest.  Block order is the caller's.
+ */
+#ifndef WXO_H
+#define WXO_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+  int nv;            /* 5 (hydro) or 8 (MHD) */
+  int mhd;           /* active magnetic field (parameters `mhd`) */
+  int solver;        /* SOLVER_HLL1=2, HLL=3, HLLC=4, HLLE=5, HLLD=6 */
+  int limiter;       /* LIMITER_* (-1..6) */
+  int eight_wave;    /* Powell source term */
+  int cooling;       /* 0 = none, 1 = COOL_TABLE */
+  int eos;           /* EOS_ADIABATIC=0, SINGLE_SPECIE=1, TWOTEMP=2 */
+  int bc[6];         /* BC_REFLECTIVE=1, FREE_FLOW=2, PERIODIC=3 per face */
+  double gamma, cv, cfl, visc_eta;
+  double cooling_limit, mu0, mui, ion_thres;
+  double d_sc, e_sc, t_sc, p_sc, l_sc, v_sc;
+  int ntab;                  /* cooling table points */
+  const double *tab_logT;    /* ntab */
+  const double *tab_loglam;  /* ntab, log10(Lambda) */
+  int nwind;
+  const double *winds;       /* 8 per wind: xc,yc,zc,R,mdot,vinf,T,mu (cgs) */
+} wxo_config;
+
+typedef struct wxo_sim wxo_sim;
+
+/* Mesh: nroot root blocks per dimension, maxlev levels, dx[3*maxlev]
+   (dx(l), then dy(l), then dz(l)), nblk active blocks with reference bIDs. */
+wxo_sim *wxo_create(const wxo_config *cfg, const int nroot[3], int maxlev,
+                    const double *dxyz, int nblk, const int *bids);
+void wxo_destroy(wxo_sim *s);
+
+/* Interior conserved state [nblk][nv][16][16][16]; ghosts zeroed, then
+   PRIM = flow2prim(U) on all cells (updatePrims). */
+void wxo_set_state(wxo_sim *s, const double *u_interior);
+/* Interior U and PRIM out (either may be NULL). */
+void wxo_get_state(const wxo_sim *s, double *u_interior, double *prim_interior);
+
+/* One iteration of the main loop: it is incremented first; dt, time and
+   nextout are read and updated (ramp, output clamp).  tout_scale =
+   dtout/t_sc.  Returns the number of floored cells. */
+long wxo_step(wxo_sim *s, int *it, double *dt, double *time, int *nextout,
+              double dtout, double t_sc);
+
+/* Pieces of a step (for finer-grained parity checks). */
+double wxo_timestep_raw(const wxo_sim *s);   /* CFL * min over cells */
+void wxo_boundary(wxo_sim *s, int depth, int which /*0=U,1=UP*/);
+long wxo_calc_prims(wxo_sim *s, int which, int cells /*0 ALL,1 PHYS,2 GHOST*/);
+void wxo_godunov(wxo_sim *s, int order, double dt);
+void wxo_viscous(wxo_sim *s);
+void wxo_cooling(wxo_sim *s, double dt);
+/* raw arrays with ghosts, [nblk][nv][20][20][20]; which: 0=U,1=UP,2=PRIM */
+double *wxo_array(wxo_sim *s, int which);
+
+/* Per-function entry points (single cell / single interface). */
+int wxo_flow2prim(const wxo_config *c, const double *u, double *p);
+void wxo_prim2flow(const wxo_config *c, const double *p, double *u);
+void wxo_prim2fluxes(const wxo_config *c, const double *p, int dim, double *f);
+void wxo_limiter(const double *pll, double *pl, double *pr, const double *prr, int lim, int n);
+void wxo_riemann(const wxo_config *c, int solver, const double *pl, const double *pr, double *f);
+void wxo_speeds(const wxo_config *c, const double *p, double out[5]);
+double wxo_find_coef(const wxo_config *c, double logT);
+double wxo_calc_temp(const wxo_config *c, const double *p);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,97 @@
+"""Run the reference oracle binary (oracle/_ref/<workload>/harness) on a
+workload's initial condition and read its dumps back.  Used only by
+tests/golden/make_golden.py in the build container, where the reference
+sources exist; the GPU tests read the committed fixtures instead."""
+from __future__ import annotations
+
+import os
+import struct
+import subprocess
+import tempfile
+
+import numpy as np
+
+from walicxe3d_amd import workloads as W
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+REFDIR = os.path.join(ROOT, "oracle", "_ref")
+
+
+def harness_path(name: str) -> str:
+    return os.path.join(REFDIR, name, "harness")
+
+
+def write_ic_with_passive(path, w, blocks, U):
+    """The oracle builds carry one passive scalar (zero everywhere)."""
+    nb = U.shape[0]
+    U1 = np.concatenate([U, np.zeros((nb, 1) + U.shape[2:])], axis=1)
+    refine = w.refine_order()
+    wp = W.wind_params(w)
+    with open(path, "wb") as f:
+        f.write(struct.pack("<5i", W.IC_MAGIC, w.nv + 1, len(refine), nb, len(wp)))
+        if refine:
+            f.write(np.asarray(refine, dtype="<i4").tobytes())
+        for rec in wp:
+            f.write(np.asarray(rec, dtype="<f8").tobytes())
+        for n, b in enumerate(blocks):
+            f.write(struct.pack("<i", b))
+            f.write(np.ascontiguousarray(U1[n], dtype="<f8").tobytes())
+
+
+def read_topology(path):
+    with open(path, "rb") as f:
+        nact, maxlev, nbx, nby, nbz = struct.unpack("<5i", f.read(20))
+        rec = np.frombuffer(f.read(8 * nact), dtype="<i4").reshape(nact, 2)
+        dx = np.frombuffer(f.read(8 * 3 * maxlev), dtype="<f8").reshape(3, maxlev)
+        consts = np.frombuffer(f.read(80), dtype="<f8")
+    return dict(bids=rec[:, 0].copy(), levels=rec[:, 1].copy(), maxlev=maxlev,
+                nroot=(nbx, nby, nbz), dx=dx.copy(),
+                consts=dict(zip(["gamma", "cv", "cfl", "visc", "d_sc", "e_sc", "t_sc", "p_sc",
+                                 "l_sc", "v_sc"], consts.tolist())))
+
+
+def read_state(path, nblk, nv_ref):
+    a = np.fromfile(path, dtype="<f8")
+    return a.reshape(nblk, nv_ref, 16, 16, 16)
+
+
+def run(w, nsteps, dumps, workdir=None, unit_op=None, unit_in=None, timeout=3600):
+    """Run the harness; returns dict with topo, steps (it, dt, time, tout),
+    and states {step: (U, PRIM)} with the passive scalar stripped."""
+    exe = harness_path(w.name)
+    if not os.path.exists(exe):
+        raise FileNotFoundError(exe)
+    own = workdir is None
+    wd = workdir or tempfile.mkdtemp(prefix="wxref_")
+    if w.cooling:
+        lt, lam = W.cooling_table()
+        W.write_cooling_table(os.path.join(wd, "cooltable.dat"), lt, lam)
+    if unit_op is not None:
+        with open(os.path.join(wd, "harness.nml"), "w") as f:
+            f.write("&ctl\n mode='unit'\n op=%d\n/\n" % unit_op)
+        with open(os.path.join(wd, "unit_in.bin"), "wb") as f:
+            f.write(struct.pack("<i", unit_in.shape[0]))
+            f.write(np.ascontiguousarray(unit_in, dtype="<f8").tobytes())
+        subprocess.run([exe], cwd=wd, check=True, timeout=timeout, capture_output=True)
+        return np.fromfile(os.path.join(wd, "unit_out.bin"), dtype="<f8")
+    slots, blocks, U = W.initial_state(w)
+    write_ic_with_passive(os.path.join(wd, "ic.bin"), w, blocks, U)
+    dl = ",".join(str(d) for d in dumps)
+    with open(os.path.join(wd, "harness.nml"), "w") as f:
+        f.write("&ctl\n mode='run'\n nsteps=%d\n ndump=%d\n dumps=%s\n/\n" % (nsteps, len(dumps), dl))
+    r = subprocess.run([exe], cwd=wd, timeout=timeout, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("reference harness failed (%d): %s %s" % (r.returncode, r.stdout[-2000:], r.stderr[-2000:]))
+    topo = read_topology(os.path.join(wd, "topo.bin"))
+    steps = np.loadtxt(os.path.join(wd, "steps.txt"), ndmin=2)
+    nblk = len(topo["bids"])
+    states = {}
+    for d in dumps:
+        u = read_state(os.path.join(wd, "u_%05d.bin" % d), nblk, w.nv + 1)[:, : w.nv]
+        p = read_state(os.path.join(wd, "prim_%05d.bin" % d), nblk, w.nv + 1)[:, : w.nv]
+        states[d] = (u.copy(), p.copy())
+    out = dict(topo=topo, steps=steps, states=states, ic_blocks=blocks, ic_U=U)
+    if own:
+        import shutil
+        shutil.rmtree(wd, ignore_errors=True)
+    return out
