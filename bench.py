@@ -1,0 +1,206 @@
+#!/usr/bin/env python3
+"""Benchmark: zone-updates/s of the fused FP32 form against the plain FP64
+form on the same GPU, with the L1 error of every variable against FP64, the
+HBM roofline fraction of the dominant kernel and a CPU baseline.
+
+A step is one full timestep (two stages, viscosity, primitives, cooling, dt)
+of the whole block list; a zone-update is one interior cell advanced one
+step.  Default workload: Orszag-Tang vortex 256^3 (4096 blocks of 16^3,
+HLLD, periodic).  Inputs are resident in HBM when the timed region starts.
+
+    python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--workload ot256]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "zone-updates/s/GPU (batched FP32 vs plain FP64, same GPU) at stated L1 err; % HBM BW"
+HBM_PEAK = 8.0e12  # MI355X HBM3E, B/s
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def traffic_from_profiles(workload):
+    """HBM bytes per stage-2 launch measured with rocprofv3 PMC counters
+    (summary committed under profiles/), or None."""
+    path = os.path.join(ROOT, "profiles", "stage2_%s_traffic.txt" % workload)
+    if not os.path.exists(path):
+        return None
+    vals = {}
+    for line in open(path):
+        if "=" in line and not line.lstrip().startswith("#"):
+            k, v = line.split("=", 1)
+            try:
+                vals[k.strip()] = float(v.split()[0])
+            except ValueError:
+                pass
+    return vals.get("hbm_bytes_per_launch")
+
+
+def cpu_baseline(budget_s=20.0):
+    """The reference itself (oracle/_ref, compiled Walicxe3D sources, one MPI
+    rank) on the 64^3 Orszag-Tang workload, timed on this host."""
+    sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
+    from walicxe3d_amd import workloads as W
+    w = W.get("ot64")
+    try:
+        import refrun
+        exe = refrun.harness_path(w.name)
+        if not os.path.exists(exe):
+            raise FileNotFoundError(exe)
+        nsteps = 4
+        t0 = time.time()
+        refrun.run(w, 1, [])            # start-up cost (init, IC read)
+        t_init = time.time() - t0
+        t0 = time.time()
+        refrun.run(w, nsteps + 1, [])
+        t = time.time() - t0 - t_init
+        zones = len(w.blocks()) * 4096 * nsteps
+        return {"value": zones / t, "unit": "zone-updates/s", "cores": 1, "kind": "reference",
+                "sample": "Walicxe3D reference (oracle/_ref, amdflang -O2, 1 MPI rank) on Orszag-Tang 64^3 "
+                          "(64 blocks, HLLD), %d steps, start-up subtracted" % nsteps}
+    except Exception as e:  # reference binary not available: time the C restatement
+        log("reference binary unavailable (%s); timing the C restatement" % e)
+        from oracle import wxo
+        slots, blocks, U0 = W.initial_state(w)
+        s = wxo.Sim(w, blocks)
+        s.set_state(U0)
+        t0 = time.time()
+        n = 0
+        while time.time() - t0 < budget_s and n < 40:
+            s.step()
+            n += 1
+        t = time.time() - t0
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        return {"value": len(blocks) * 4096 * n / t, "unit": "zone-updates/s", "cores": threads, "kind": "port",
+                "sample": "C restatement (OpenMP) on Orszag-Tang 64^3, %d steps" % n}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="ot256")
+    ap.add_argument("--err-steps", type=int, default=10)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the block list of this path lives on a single device "
+              "(multi-GPU sharding is not built); run with --gpus 1", file=sys.stderr)
+        return 2
+
+    from walicxe3d_amd import hip
+    from walicxe3d_amd import workloads as W
+
+    w = W.get(args.workload)
+    t0 = time.time()
+    slots, blocks, U0 = W.initial_state(w)
+    bids = np.asarray(blocks, dtype=np.int32)
+    nblk = len(bids)
+    zones = nblk * 4096
+    log("workload %s: %d blocks, %d zones, IC %.1fs" % (w.name, nblk, zones, time.time() - t0))
+
+    # ---- fused FP32 form ----------------------------------------------------
+    f = hip.Walicxe3D(w, form=hip.FUSED_FP32, bids=bids)
+    f.upload(U0)
+    f.run(args.warmup)
+    t0 = time.perf_counter()
+    f.run(args.steps)                      # returns after a stream synchronize
+    t_fused = time.perf_counter() - t0
+    fused_zps = zones * args.steps / t_fused
+    s1, s2 = f.time_stages(3)
+    log("fused: %.3f ms/step, %.3e zone-updates/s; stage1 %.3f ms, stage2 %.3f ms"
+        % (1e3 * t_fused / args.steps, fused_zps, s1, s2))
+    f.close()
+
+    # ---- plain FP64 form ----------------------------------------------------
+    plain = None
+    if not args.no_plain:
+        p = hip.Walicxe3D(w, form=hip.PLAIN_FP64, bids=bids)
+        p.upload(U0)
+        for _ in range(args.warmup):
+            p.step()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            p.step()
+        t_plain = time.perf_counter() - t0
+        plain_zps = zones * args.steps / t_plain
+        log("plain: %.3f ms/step, %.3e zone-updates/s" % (1e3 * t_plain / args.steps, plain_zps))
+        p.close()
+        plain = {"value": plain_zps, "ms_per_step": 1e3 * t_plain / args.steps, "dtype": "f64",
+                 "speedup_fused_over_plain": fused_zps / plain_zps}
+
+    # ---- L1 error of the fused form against FP64 after err-steps steps -----
+    err = None
+    if not args.no_plain:
+        a = hip.Walicxe3D(w, form=hip.FUSED_FP32, bids=bids)
+        a.upload(U0)
+        a.run(args.err_steps)
+        ua = a.download(prim=False)
+        a.close()
+        b = hip.Walicxe3D(w, form=hip.PLAIN_FP64, bids=bids)
+        b.upload(U0)
+        for _ in range(args.err_steps):
+            b.step()
+        ub = b.download(prim=False)
+        b.close()
+        d = np.abs(ua - ub).sum(axis=(0, 2, 3, 4))
+        s = np.abs(ub).sum(axis=(0, 2, 3, 4))
+        rel = np.where(s > 0, d / np.maximum(s, 1e-300), d)
+        names = ["rho", "mx", "my", "mz", "E", "bx", "by", "bz"][: w.nv]
+        err = {"steps": args.err_steps, "reference": "plain FP64 GPU form (bit-exact with the FP64 oracle)",
+               "rel_l1": {n: float("%.3e" % v) for n, v in zip(names, rel)}, "max": float("%.3e" % rel.max())}
+        log("fused rel L1 vs FP64 after %d steps: %s" % (args.err_steps, err["rel_l1"]))
+
+    # ---- roofline of the dominant kernel (stage 2) ---------------------------
+    nv = w.nv
+    b_stage2 = zones * nv * (4 + 8 + 8)      # read UP (f32) + U (f64), write U (f64)
+    b_stage1 = zones * nv * (8 + 4)          # read U (f64), write UP (f32)
+    dom_ms, dom_bytes, dom = (s2, b_stage2, "stage2") if s2 >= s1 else (s1, b_stage1, "stage1")
+    achieved = dom_bytes / (dom_ms * 1e-3)
+    roof = {"bound": "hbm", "kernel": "k_stage<%d,%s>" % (nv, "S2" if dom == "stage2" else "S1"),
+            "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": achieved / HBM_PEAK,
+            "traffic": traffic_from_profiles(w.name), "algorithmic_bytes_per_launch": dom_bytes,
+            "avg_launch_ms": dom_ms}
+    step_bytes = zones * nv * 32
+    cpu = None if args.no_cpu else cpu_baseline()
+    out = {
+        "metric": METRIC,
+        "value": fused_zps,
+        "unit": "zone-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * t_fused / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "%s: %s" % (w.name, w.description), "blocks": nblk, "zones": zones,
+                   "form": "fused FP32 bulk, FP64 conserved accumulation", "solver": w.solver,
+                   "limiter": w.limiter, "step_hbm_roofline_frac": (fused_zps * step_bytes / zones) / HBM_PEAK},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "plain_fp64": plain,
+        "l1_err_vs_fp64": err,
+    }
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

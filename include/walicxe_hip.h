@@ -1,0 +1,149 @@
+/*
+ * walicxe_hip.h - C ABI of the MI355X (gfx950) implementation of
+ * Walicxe3D's per-block Godunov update.
+ *
+ * Plain C types only; every call returns an int status:
+ *    0            success
+ *   >0            success, with that many cells floored (pressure/density)
+ *   WX_E* (<0)    failure (see wx_last_error for text)
+ * The device context is opaque.  One context drives one GPU (the current HIP
+ * device at wx_init).  Calls on one context must not overlap.
+ *
+ * Block state is exchanged either in the library layout (WX_LAYOUT_BLOCKS:
+ * [nblk][nv][16][16][16], i fastest, blocks in the order given to
+ * wx_set_mesh) or in the reference's Fortran layout (WX_LAYOUT_FORTRAN:
+ * U(nbMaxProc, nv, -1:18, -1:18, -1:18), block slot fastest, with the
+ * slot->bID map `localBlocks`).
+ *
+ * Each entry point names the reference routine whose interface it replaces
+ * (paths relative to the Walicxe3D source tree).
+ */
+#ifndef WALICXE_HIP_H
+#define WALICXE_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define WX_OK 0
+#define WX_EINVAL (-1)
+#define WX_EHIP (-2)
+#define WX_ENOMEM (-3)
+#define WX_ESTATE (-4)
+#define WX_ETOPO (-5)
+
+/* numerical forms */
+#define WX_FORM_PLAIN_FP64 0   /* one kernel per reference routine, FP64, reference order */
+#define WX_FORM_FUSED_FP32 1   /* one fused kernel per stage, FP32 bulk, FP64 accumulation */
+
+#define WX_LAYOUT_BLOCKS 0
+#define WX_LAYOUT_FORTRAN 1
+
+/* which array */
+#define WX_U 0
+#define WX_UP 1
+/* cell ranges (source/constants.f90:119-121) */
+#define WX_CELLS_ALL 0
+#define WX_CELLS_PHYS 1
+#define WX_CELLS_GHOST 2
+
+/* Compile-time parameters of the reference (source/parameters.f90) as a
+   run-time struct.  Enumerations use the reference's values
+   (source/constants.f90:52-104). */
+typedef struct wx_config {
+  int nv;            /* neqtot: 5 hydro, 8 MHD (npassive = 0) */
+  int mhd;           /* parameters `mhd` */
+  int solver;        /* SOLVER_HLL1=2, HLL=3, HLLC=4, HLLE=5, HLLD=6 */
+  int limiter;       /* LIMITER_NO_AVERAGE=-1 ... SUPERBEE=6 */
+  int eight_wave;    /* Powell divergence source (sources.f90) */
+  int cooling;       /* COOL_NONE=0, COOL_TABLE=1 */
+  int eos;           /* EOS_ADIABATIC=0, SINGLE_SPECIE=1, TWOTEMP=2 */
+  int bc[6];         /* LEFT..TOP: BC_REFLECTIVE=1, FREE_FLOW=2, PERIODIC=3 */
+  double gamma, cv, cfl, visc_eta;
+  double cooling_limit, mu0, mui, ion_thres;
+  double d_sc, e_sc, t_sc, p_sc, l_sc, v_sc;
+  int ntab;                  /* cooling table (loadcooldata, cooling.f90:135) */
+  const double *tab_logT;
+  const double *tab_loglam;  /* log10(Lambda) */
+  int nwind;                 /* spherical wind sources re-imposed at every */
+  const double *winds;       /* boundary call (winds.f90:123): 8 per source:
+                                xc,yc,zc,radius,mdot,vinf,temp,mu (cgs)   */
+  int form;                  /* WX_FORM_* */
+} wx_config;
+
+/* Set-up.  Replaces the allocation part of initmain (source/init.f90:41-480). */
+int wx_init(const wx_config *cfg, void **ctx);
+int wx_finalize(void *ctx);
+const char *wx_last_error(void);
+/* Library version string. */
+const char *wx_version(void);
+
+/* Mesh: root blocks per dimension, number of levels, grid spacings
+   dxyz[3*maxlev] (dx(1..maxlev), dy(..), dz(..): basegrid,
+   source/init.f90:726-730) and the active block IDs.  Builds the face
+   neighbour table that source/amr.f90:539-631 `neighbors` computes per
+   query.  Must be called again whenever the block list changes. */
+int wx_set_mesh(void *ctx, const int nroot[3], int maxlev, const double *dxyz, int nblk,
+                const int *bids);
+
+/* State transfer.  Upload sets U's interior, zeroes its ghosts and computes
+   PRIM (updatePrims, source/hydro_core.f90:43-59).  For WX_LAYOUT_FORTRAN
+   pass nbmax = nbMaxProc and local_blocks = localBlocks; slots whose bID is
+   not in the mesh are ignored. */
+int wx_upload_state(void *ctx, const double *u, int layout, int nbmax, const int *local_blocks);
+/* Download interior U and PRIM (either may be NULL). */
+int wx_download_state(void *ctx, double *u, double *prim, int layout, int nbmax,
+                      const int *local_blocks);
+
+/* getTimestep (source/hydro_solver.f90:129-211) without the start-up ramp
+   and output clamp, which need `it`, `time` and `nextout` (host scalars):
+   dt_raw = CFL * min over interior cells of dx/(|v|+c). */
+int wx_get_timestep(void *ctx, double *dt_raw);
+/* Godunov(order) (source/godunov.f90:43-190): boundary, primitives, fluxes
+   and conservative update of every block, one or two stages. */
+int wx_godunov(void *ctx, int order, double dt);
+/* doViscousStep (source/hydro_solver.f90:217-250) without `time += dt`. */
+int wx_viscous_step(void *ctx);
+/* updatePrims (source/hydro_core.f90:43-59). */
+int wx_update_prims(void *ctx);
+/* cooling (source/cooling.f90:37-118), COOL_TABLE. */
+int wx_cooling(void *ctx, double dt);
+/* One iteration of the main loop (source/main.f90:81-100): it += 1, dt from
+   getTimestep with ramp and clamp (tout = nextout*dtout/t_sc), hydroSolver,
+   updatePrims, cooling; advances *time and, when the clamp fired,
+   *nextout.  dtout and t_sc are the reference's parameters. */
+int wx_step(void *ctx, int *it, double *dt, double *time, int *nextout, double dtout, double t_sc);
+/* Advance nsteps iterations with no host synchronisation between steps (the
+   ramp and clamp run on the device); fused form only, captured in a HIP
+   graph per mesh. */
+int wx_run(void *ctx, int nsteps, int *it, double *dt, double *time, int *nextout, double dtout,
+           double t_sc);
+
+/* Fine-grained entry points for parity tests of single routines. */
+/* boundary(depth, uvars) (source/boundary.f90:46-64), which = WX_U|WX_UP */
+int wx_boundary(void *ctx, int depth, int which);
+/* calcPrimsAll(uvars, PRIM, cells) (source/hydro_core.f90:67-101) */
+int wx_calc_prims(void *ctx, int which, int cells);
+/* HLL*fluxes(b, order) + upwindStep(b, dtp) for every block
+   (source/godunov.f90:95-125,199-277) */
+int wx_fluxes_update(void *ctx, int order, double dtp);
+/* Raw device arrays with ghosts ([nblk][nv][20][20][20]) copied to the host;
+   which = 0 U, 1 UP, 2 PRIM.  Plain form only. */
+int wx_copy_raw(void *ctx, int which, double *out);
+
+/* Tile/launch choice of the fused form (autotuner): kind 0 = stage kernel
+   block size, value as documented in the implementation. */
+int wx_set_tuning(void *ctx, int kind, int value);
+/* Advances `reps` more iterations launching each stage kernel on its own
+   between HIP events on the library's stream (fused form; no reference
+   counterpart: measurement only).  The step scalars advance on the device
+   exactly as in wx_run. */
+int wx_time_stages(void *ctx, int reps);
+/* Average stage-kernel durations (ms) of the last wx_time_stages and the
+   average time per step of the last wx_run; zero if none. */
+int wx_last_timing(void *ctx, double *stage1_ms, double *stage2_ms, double *total_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -14,6 +14,8 @@
 !   mode   = 'unit' : evaluates one reference routine on every record of
 !                     unit_in.bin and writes unit_out.bin (op codes below).
 !   nsteps, ndump, dumps(:)  - steps after which U is written (0 = initial)
+!   tstop  - if > 0, stop (and write the state) after the first step that
+!            reaches time >= tstop
 !
 ! Output of 'run': topo.bin, steps.txt, u_NNNNN.bin, prim_NNNNN.bin
 !
@@ -41,7 +43,8 @@ program wx_oracle_harness
   integer :: nsteps = 0, ndump = 0
   integer :: dumps(64) = -1
   integer :: op = 0
-  namelist /ctl/ mode, nsteps, ndump, dumps, op
+  real(8) :: tstop = -1.0
+  namelist /ctl/ mode, nsteps, ndump, dumps, op, tstop
 
   integer :: s
 
@@ -76,6 +79,10 @@ program wx_oracle_harness
       dumpout = .false.
     end if
     if (any(dumps(1:max(ndump,1)) == s)) call write_state (s)
+    if (tstop > 0.0 .and. time >= tstop) then
+      if (.not. any(dumps(1:max(ndump,1)) == s)) call write_state (s)
+      exit
+    end if
   end do
   close (71)
 

@@ -1,0 +1,201 @@
+#!/usr/bin/env python3
+"""Writes the golden fixtures of tests/golden/ from the reference oracle
+binaries (oracle/_ref/<workload>/harness, built by oracle/build_ref.py from
+the unmodified Walicxe3D sources).  Run in the build container only:
+
+    python oracle/build_ref.py && python tests/golden/make_golden.py
+
+Fixtures (numpy .npz, no pickles):
+  traj_<workload>.npz   per-step (it, dt, time); at each dumped step the
+                        per-block per-variable sum and sum of |x| of U, and a
+                        seeded sample of 4096 full cell vectors of U and PRIM,
+                        keyed by reference block ID; a checksum of the initial
+                        condition so that drift of the generator is caught.
+  unit_<build>.npz      reference routine outputs on seeded inputs
+                        (flow2prim, prim2flow, prim2fluxes, limiter, Riemann
+                        solvers, wave speeds, find_coef, calcTemp).
+"""
+from __future__ import annotations
+
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, HERE)
+
+import refrun  # noqa: E402
+from walicxe3d_amd import workloads as W  # noqa: E402
+
+# steps run per workload and the steps dumped
+TRAJ = {
+    "sedov": (100, [10, 100]),
+    "briowu": (None, None),          # to t = tfin
+    "ot64": (20, [5, 20]),
+    "blast_reflect": (30, [30]),
+    "mhd_hlle": (20, [20]),
+    "amr_small": (8, [2, 8]),
+    "snr_amr4": (5, [5]),
+    "winds4": (5, [5]),
+}
+NSAMPLE = 4096
+
+
+def summarize(w, bids, U, P, seed):
+    nb = U.shape[0]
+    sums = U.reshape(nb, w.nv, -1).sum(axis=2)
+    abss = np.abs(U).reshape(nb, w.nv, -1).sum(axis=2)
+    rng = np.random.default_rng(seed)
+    bsel = rng.integers(0, nb, NSAMPLE)
+    csel = rng.integers(0, 4096, NSAMPLE)
+    su = U.reshape(nb, w.nv, -1)[bsel, :, csel]
+    sp = P.reshape(nb, w.nv, -1)[bsel, :, csel]
+    return dict(sums=sums, abss=abss, sample_bid=np.asarray(bids)[bsel].astype(np.int32),
+                sample_cell=csel.astype(np.int32), sample_u=su, sample_p=sp)
+
+
+def make_traj(name):
+    w = W.get(name)
+    n, dumps = TRAJ[name]
+    t0 = time.time()
+    if n is None:
+        # Brio-Wu: run until the output clamp lands exactly on t = tfin
+        r = refrun.run(w, 5000, [50], timeout=7200, tstop=w.tfin)
+        dumps = r["dumps"]
+        n = int(r["steps"][-1, 0])
+    else:
+        r = refrun.run(w, n, dumps, timeout=7200)
+    out = dict(steps=r["steps"], bids=r["topo"]["bids"].astype(np.int32),
+               levels=r["topo"]["levels"].astype(np.int32), dumps=np.asarray(dumps, dtype=np.int32),
+               ic_sum=np.asarray([np.sum(r["ic_U"]), np.sum(np.abs(r["ic_U"]))]),
+               dx=r["topo"]["dx"])
+    for d in dumps:
+        u, p = r["states"][d]
+        for k, v in summarize(w, r["topo"]["bids"], u, p, 1000 * w.seed + d).items():
+            out["d%d_%s" % (d, k)] = v
+    np.savez_compressed(os.path.join(HERE, "traj_%s.npz" % name), **out)
+    print("traj %-14s steps=%d dumps=%s (%.1fs)" % (name, n, dumps, time.time() - t0), flush=True)
+
+
+def rand_prims(rng, n, nv, mhd):
+    p = np.empty((n, nv))
+    p[:, 0] = np.exp(rng.uniform(-3, 2, n))
+    p[:, 1:4] = rng.normal(0, 1.5, (n, 3))
+    p[:, 4] = np.exp(rng.uniform(-4, 2, n))
+    if nv == 8:
+        p[:, 5:8] = rng.normal(0, 1.0, (n, 3))
+        z = rng.random(n) < 0.1
+        p[z, 5] = 0.0
+    return p
+
+
+def make_unit(name, ops):
+    """Reference routine outputs for the build of workload `name`."""
+    w = W.get(name)
+    nv = w.nv
+    rng = np.random.default_rng(77 + w.seed)
+    n = 3000
+    out = {}
+    nvr = nv + 1  # oracle builds carry one passive scalar
+
+    def pad(a):
+        return np.concatenate([a, np.zeros((a.shape[0], 1))], axis=1)
+
+    for op in ops:
+        if op == 1:
+            pr = rand_prims(rng, n, nv, w.mhd)
+            u = np.array([_p2u(w, x) for x in pr])
+            u[::7, 4] *= rng.uniform(0.0, 1.0, u[::7].shape[0])  # drive some to the floor
+            inp = pad(u)
+            o = refrun.run(w, 0, [], unit_op=1, unit_in=inp).reshape(n, nvr + 1)
+            out["flow2prim_in"] = u
+            out["flow2prim_out"] = o[:, :nv]
+            out["flow2prim_istat"] = o[:, nvr]
+        elif op == 2:
+            pr = rand_prims(rng, n, nv, w.mhd)
+            o = refrun.run(w, 0, [], unit_op=2, unit_in=pad(pr)).reshape(n, nvr)
+            out["prim2flow_in"], out["prim2flow_out"] = pr, o[:, :nv]
+        elif op == 3:
+            pr = rand_prims(rng, n, nv, w.mhd)
+            dims = rng.integers(1, 4, n).astype(np.float64)
+            inp = np.concatenate([pad(pr), dims[:, None]], axis=1)
+            o = refrun.run(w, 0, [], unit_op=3, unit_in=inp).reshape(n, nvr)
+            out["prim2fluxes_in"], out["prim2fluxes_dim"], out["prim2fluxes_out"] = pr, dims, o[:, :nv]
+        elif op == 4:
+            q = rng.normal(0, 1, (n, 4, nv))
+            q[::5, 1] = q[::5, 0]           # flat slopes
+            q[1::5, 2] = q[1::5, 1]
+            lim = rng.integers(-1, 7, n).astype(np.float64)
+            inp = np.concatenate([pad(q[:, 0]), pad(q[:, 1]), pad(q[:, 2]), pad(q[:, 3]), lim[:, None]], axis=1)
+            o = refrun.run(w, 0, [], unit_op=4, unit_in=inp).reshape(n, 2, nvr)
+            out["limiter_in"], out["limiter_lim"] = q, lim
+            out["limiter_out"] = o[:, :, :nv]
+        elif op in (5, 6, 7, 8):
+            pl = rand_prims(rng, n, nv, w.mhd)
+            pr = rand_prims(rng, n, nv, w.mhd)
+            # a share of near-identical and strongly supersonic pairs covers every branch
+            pr[::6] = pl[::6] * (1 + 1e-3 * rng.normal(size=(pr[::6].shape)))
+            pl[1::6, 1] += 8.0
+            pr[1::6, 1] += 8.0
+            pl[2::6, 1] -= 8.0
+            pr[2::6, 1] -= 8.0
+            if nv == 8:
+                pr[3::6, 5] = pl[3::6, 5]
+            inp = np.concatenate([pad(pl), pad(pr)], axis=1)
+            o = refrun.run(w, 0, [], unit_op=op, unit_in=inp).reshape(n, nvr)
+            key = {5: "hll", 6: "hllc", 7: "hlle", 8: "hlld"}[op]
+            out[key + "_pl"], out[key + "_pr"], out[key + "_out"] = pl, pr, o[:, :nv]
+        elif op == 9:
+            pr = rand_prims(rng, n, nv, w.mhd)
+            o = refrun.run(w, 0, [], unit_op=9, unit_in=pad(pr)).reshape(n, 5)
+            out["speeds_in"], out["speeds_out"] = pr, o
+        elif op == 10:
+            lt, lam = W.cooling_table()
+            x = np.concatenate([rng.uniform(3.5, 8.8, n - len(lt) - 2), lt, [lt[-1] + 0.3, 3.9]])
+            o = refrun.run(w, 0, [], unit_op=10, unit_in=x[:, None])
+            out["find_coef_in"], out["find_coef_out"] = x, o
+        elif op == 11:
+            pr = rand_prims(rng, n, nv, w.mhd)
+            o = refrun.run(w, 0, [], unit_op=11, unit_in=pad(pr))
+            out["calctemp_in"], out["calctemp_out"] = pr, o
+    np.savez_compressed(os.path.join(HERE, "unit_%s.npz" % name), **out)
+    print("unit %-14s ops=%s" % (name, ops), flush=True)
+
+
+def _p2u(w, p):
+    u = np.empty_like(p)
+    u[0] = p[0]
+    u[1:4] = p[0] * p[1:4]
+    u[4] = 0.5 * p[0] * (p[1] ** 2 + p[2] ** 2 + p[3] ** 2) + w.cv * p[4]
+    if w.nv == 8:
+        u[4] += 0.5 * (p[5] ** 2 + p[6] ** 2 + p[7] ** 2)
+        u[5:8] = p[5:8]
+    return u
+
+
+UNITS = {
+    "sedov": [1, 2, 3, 4, 5, 6, 9],          # HD, HLLC build
+    "blast_reflect": [1, 6, 9],              # HD, gamma 1.4 (HLLC s* = 0 branch when solver is HLL)
+    "briowu": [1, 2, 3, 4, 5, 7, 8, 9],      # MHD gamma 2
+    "amr_small": [10, 11],                   # cooling table + temperature
+}
+
+
+def main(argv):
+    names = argv or list(TRAJ)
+    for n in names:
+        if n.startswith("unit:"):
+            make_unit(n[5:], UNITS[n[5:]])
+        else:
+            make_traj(n)
+    if not argv:
+        for n, ops in UNITS.items():
+            make_unit(n, ops)
+
+
+if __name__ == "__main__":
+    main(sys.argv[1:])

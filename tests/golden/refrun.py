@@ -55,7 +55,7 @@ def read_state(path, nblk, nv_ref):
     return a.reshape(nblk, nv_ref, 16, 16, 16)
 
 
-def run(w, nsteps, dumps, workdir=None, unit_op=None, unit_in=None, timeout=3600):
+def run(w, nsteps, dumps, workdir=None, unit_op=None, unit_in=None, timeout=3600, tstop=-1.0):
     """Run the harness; returns dict with topo, steps (it, dt, time, tout),
     and states {step: (U, PRIM)} with the passive scalar stripped."""
     exe = harness_path(w.name)
@@ -78,7 +78,8 @@ def run(w, nsteps, dumps, workdir=None, unit_op=None, unit_in=None, timeout=3600
     write_ic_with_passive(os.path.join(wd, "ic.bin"), w, blocks, U)
     dl = ",".join(str(d) for d in dumps)
     with open(os.path.join(wd, "harness.nml"), "w") as f:
-        f.write("&ctl\n mode='run'\n nsteps=%d\n ndump=%d\n dumps=%s\n/\n" % (nsteps, len(dumps), dl))
+        f.write("&ctl\n mode='run'\n nsteps=%d\n ndump=%d\n dumps=%s\n tstop=%r\n/\n"
+                % (nsteps, len(dumps), dl or "-1", float(tstop)))
     r = subprocess.run([exe], cwd=wd, timeout=timeout, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("reference harness failed (%d): %s %s" % (r.returncode, r.stdout[-2000:], r.stderr[-2000:]))
@@ -86,11 +87,13 @@ def run(w, nsteps, dumps, workdir=None, unit_op=None, unit_in=None, timeout=3600
     steps = np.loadtxt(os.path.join(wd, "steps.txt"), ndmin=2)
     nblk = len(topo["bids"])
     states = {}
+    if tstop > 0 and len(steps) and int(steps[-1, 0]) not in dumps:
+        dumps = list(dumps) + [int(steps[-1, 0])]
     for d in dumps:
         u = read_state(os.path.join(wd, "u_%05d.bin" % d), nblk, w.nv + 1)[:, : w.nv]
         p = read_state(os.path.join(wd, "prim_%05d.bin" % d), nblk, w.nv + 1)[:, : w.nv]
         states[d] = (u.copy(), p.copy())
-    out = dict(topo=topo, steps=steps, states=states, ic_blocks=blocks, ic_U=U)
+    out = dict(topo=topo, steps=steps, states=states, ic_blocks=blocks, ic_U=U, dumps=list(dumps))
     if own:
         import shutil
         shutil.rmtree(wd, ignore_errors=True)

@@ -1,0 +1,61 @@
+"""The C ABI: the library loads and exports every entry point declared in
+include/walicxe_hip.h; argument checks that need no GPU."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+from conftest import ROOT
+from walicxe3d_amd import hip
+
+HEADER = os.path.join(ROOT, "include", "walicxe_hip.h")
+
+
+def declared():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(wx_\w+)\s*\(", text, flags=re.M)
+
+
+def test_header_declares_entry_points():
+    names = declared()
+    assert len(names) >= 20
+    assert set(names) == set(hip.SIGNATURES), set(names) ^ set(hip.SIGNATURES)
+
+
+def test_library_exports_every_symbol():
+    if not os.path.exists(hip.LIBPATH):
+        pytest.skip("library not built")
+    lib = C.CDLL(hip.LIBPATH)
+    for n in declared():
+        assert hasattr(lib, n), n
+
+
+def test_version_and_error_strings():
+    if not os.path.exists(hip.LIBPATH):
+        pytest.skip("library not built")
+    L = hip.load()
+    assert b"gfx950" in L.wx_version()
+    assert isinstance(L.wx_last_error(), bytes)
+
+
+def test_rejects_bad_config_before_touching_a_device():
+    """Argument validation runs before any device query."""
+    if not os.path.exists(hip.LIBPATH):
+        pytest.skip("library not built")
+    from walicxe3d_amd import workloads as W
+    L = hip.load()
+    cfg = hip.make_config(W.get("sedov"))
+    cfg.nv = 6
+    h = C.c_void_p()
+    assert L.wx_init(C.byref(cfg), C.byref(h)) == -1
+    assert b"nv" in L.wx_last_error()
+    cfg = hip.make_config(W.get("sedov"))
+    cfg.solver = 6   # HLLD needs MHD
+    assert L.wx_init(C.byref(cfg), C.byref(h)) == -1
+    cfg = hip.make_config(W.get("sedov"), form=hip.FUSED_FP32)
+    cfg.solver = 2   # first-order HLL runs only in the plain form
+    assert L.wx_init(C.byref(cfg), C.byref(h)) == -1
+    assert L.wx_set_mesh(None, None, 1, None, 1, None) == -1
+    assert L.wx_step(None, None, None, None, None, 1.0, 1.0) == -1

@@ -1,0 +1,646 @@
+// wx_api.cpp - the C ABI (include/walicxe_hip.h): context, mesh topology,
+// state transfer and the step sequence of both numerical forms.
+#include "../../include/walicxe_hip.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "wx_device.h"
+#include "wx_fused.h"
+#include "wx_kernels.h"
+
+using namespace wx;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                          \
+  do {                                                                                        \
+    hipError_t e_ = (expr);                                                                   \
+    if (e_ != hipSuccess) return fail(WX_EHIP, std::string(#expr ": ") + hipGetErrorString(e_)); \
+  } while (0)
+
+struct Ctx {
+  wx_config cfg{};
+  std::vector<double> tabT, tabC, winds;
+  DevCfg dc{};
+  int device = 0;
+  hipStream_t st = nullptr;
+  // mesh
+  int nblk = 0, maxlev = 0, nroot[3] = {0, 0, 0};
+  std::vector<int> bids;
+  std::vector<BlockInfo> info;
+  std::unordered_map<int, int> index;
+  // device memory
+  BlockInfo *d_info = nullptr;
+  double *d_tabT = nullptr, *d_tabC = nullptr;
+  double *dU = nullptr, *dUP = nullptr, *dP = nullptr, *dF = nullptr, *dTmp = nullptr;
+  unsigned long long *d_scal = nullptr;  // [0] dt bits, [1] floor count
+  bool have_state = false;
+  FusedState fused;
+};
+
+void free_mesh(Ctx *c) {
+  hipFree(c->d_info);
+  hipFree(c->dU);
+  hipFree(c->dUP);
+  hipFree(c->dP);
+  hipFree(c->dF);
+  hipFree(c->dTmp);
+  c->d_info = nullptr;
+  c->dU = c->dUP = c->dP = c->dF = c->dTmp = nullptr;
+  fused_free(c->fused);
+  c->have_state = false;
+}
+
+// ---------------------------------------------------------------------------
+// Topology: the face neighbour table (source/amr.f90:439-631 neighborLevel and
+// neighbors, :831-909 siblingCoords and children, :916-997 father and
+// bcoords), built once per block list with a bID -> index map instead of the
+// reference's linear scan of the global block list per query.
+// ---------------------------------------------------------------------------
+struct Numbering {
+  int nroot[3];
+  int maxlev;
+  std::vector<long> minid, maxid;
+  void init(const int nr[3], int ml) {
+    for (int d = 0; d < 3; d++) nroot[d] = nr[d];
+    maxlev = ml;
+    long n = (long)nr[0] * nr[1] * nr[2];
+    minid.assign(ml, 0);
+    maxid.assign(ml, 0);
+    minid[0] = 1;
+    maxid[0] = n;
+    for (int l = 1; l < ml; l++) {
+      minid[l] = maxid[l - 1] + 1;
+      maxid[l] = maxid[l - 1] + n * (1L << (3 * l));
+    }
+  }
+  int level(long bid) const {
+    for (int l = 0; l < maxlev; l++)
+      if (bid >= minid[l] && bid <= maxid[l]) return l + 1;
+    return -1;
+  }
+  void nb(int l, long n[3]) const {
+    for (int d = 0; d < 3; d++) n[d] = (long)nroot[d] << (l - 1);
+  }
+  void coords(long bid, int l, int c[3]) const {
+    long n[3];
+    nb(l, n);
+    long loc = bid - minid[l - 1];
+    c[0] = (int)(loc % n[0]) + 1;
+    c[1] = (int)((loc / n[0]) % n[1]) + 1;
+    c[2] = (int)(loc / (n[0] * n[1])) + 1;
+  }
+  long bid(int l, const int c[3]) const {
+    long n[3];
+    nb(l, n);
+    return 1 + (c[0] - 1) + (long)(c[1] - 1) * n[0] + (long)(c[2] - 1) * n[0] * n[1] + minid[l - 1] - 1;
+  }
+};
+
+int build_topology(Ctx *c, const int *bids) {
+  Numbering N;
+  N.init(c->nroot, c->maxlev);
+  c->index.clear();
+  c->bids.assign(bids, bids + c->nblk);
+  for (int b = 0; b < c->nblk; b++) {
+    if (N.level(bids[b]) < 0) return fail(WX_ETOPO, "block ID outside the mesh: " + std::to_string(bids[b]));
+    if (!c->index.emplace(bids[b], b).second) return fail(WX_ETOPO, "duplicate block ID " + std::to_string(bids[b]));
+  }
+  c->info.assign(c->nblk, BlockInfo{});
+  for (int b = 0; b < c->nblk; b++) {
+    BlockInfo &B = c->info[b];
+    B.bid = bids[b];
+    B.level = N.level(bids[b]);
+    N.coords(bids[b], B.level, B.coords);
+    for (int d = 0; d < 3; d++) B.sib[d] = B.level > 1 ? (B.coords[d] - 1) % 2 : -1;
+    long nbl[3];
+    N.nb(B.level, nbl);
+    for (int f = 0; f < 6; f++) {
+      Face &F = B.face[f];
+      F.type = NEIGH_BOUNDARY;
+      for (int q = 0; q < 4; q++) F.src[q] = -1;
+      int n = f >> 1, low = (f & 1) == 0;
+      int p[3] = {B.coords[0], B.coords[1], B.coords[2]};
+      p[n] += low ? -1 : 1;
+      if (p[n] < 1 || p[n] > nbl[n]) {
+        if (c->cfg.bc[f] != BC_PERIODIC) continue;
+        p[n] = low ? (int)nbl[n] : 1;  // periodic wrap
+      }
+      long nid = N.bid(B.level, p);
+      auto it = c->index.find((int)nid);
+      if (it != c->index.end()) {
+        F.type = NEIGH_SAME;
+        F.src[0] = it->second;
+        continue;
+      }
+      if (B.level > 1) {
+        int fp[3] = {(p[0] + 1) / 2, (p[1] + 1) / 2, (p[2] + 1) / 2};
+        auto itf = c->index.find((int)N.bid(B.level - 1, fp));
+        if (itf != c->index.end()) {
+          F.type = NEIGH_COARSER;
+          F.src[0] = itf->second;
+          continue;
+        }
+      }
+      if (B.level >= c->maxlev)
+        return fail(WX_ETOPO, "no neighbour for block " + std::to_string(B.bid) + " face " + std::to_string(f));
+      int t1 = (n == 0) ? 1 : 0, t2 = (n == 2) ? 1 : 2;
+      F.type = NEIGH_FINER;
+      for (int s2 = 0; s2 < 2; s2++)
+        for (int s1 = 0; s1 < 2; s1++) {
+          int ch[3];
+          ch[n] = 2 * p[n] - 1 + (low ? 1 : 0);
+          ch[t1] = 2 * p[t1] - 1 + s1;
+          ch[t2] = 2 * p[t2] - 1 + s2;
+          auto itc = c->index.find((int)N.bid(B.level + 1, ch));
+          if (itc == c->index.end())
+            return fail(WX_ETOPO, "2:1 level balance violated at block " + std::to_string(B.bid) + " face " +
+                                      std::to_string(f));
+          F.src[s1 + 2 * s2] = itc->second;
+        }
+    }
+  }
+  return WX_OK;
+}
+
+void fill_devcfg(Ctx *c, const double *dxyz) {
+  DevCfg &d = c->dc;
+  const wx_config &g = c->cfg;
+  d.pd = Phys<double>{g.nv, g.mhd, g.solver, g.limiter, g.gamma, g.cv};
+  d.pf = Phys<float>{g.nv, g.mhd, g.solver, g.limiter, (float)g.gamma, (float)g.cv};
+  d.nv = g.nv;
+  d.eight_wave = g.eight_wave;
+  d.cooling = g.cooling;
+  d.eos = g.eos;
+  d.nwind = g.nwind;
+  d.ntab = g.ntab;
+  for (int f = 0; f < 6; f++) d.bc[f] = g.bc[f];
+  d.visc_eta = g.visc_eta;
+  d.cfl = g.cfl;
+  d.cooling_limit = g.cooling_limit;
+  d.ion_thres = g.ion_thres;
+  d.d_sc = g.d_sc;
+  d.e_sc = g.e_sc;
+  d.t_sc = g.t_sc;
+  d.p_sc = g.p_sc;
+  d.l_sc = g.l_sc;
+  d.v_sc = g.v_sc;
+  d.temp_mu0 = g.mu0 * AMU * g.p_sc / g.d_sc / KB;
+  d.temp_mui = g.mui * AMU * g.p_sc / g.d_sc / KB;
+  d.mui_amu = g.mui * AMU;
+  for (int w = 0; w < g.nwind; w++)
+    for (int q = 0; q < 8; q++) d.winds[w][q] = c->winds[8 * w + q];
+  if (dxyz) {
+    d.maxlev = c->maxlev;
+    for (int a = 0; a < 3; a++)
+      for (int l = 0; l < c->maxlev; l++) d.dx[a][l] = dxyz[a * c->maxlev + l];
+  }
+  d.nblk = c->nblk;
+  d.info = c->d_info;
+  d.tabT = c->d_tabT;
+  d.tabC = c->d_tabC;
+}
+
+__global__ void k_scatter(int nv, int nblk, const double *src, double *dst, int zero_ghosts) {
+  // one thread per stored cell: dst [blk][nv][20^3] <- src [blk][nv][16^3]
+  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t total = (size_t)nblk * nv * NCELL;
+  if (t >= total) return;
+  int q = (int)(t % NCELL);
+  size_t bv = t / NCELL;
+  int i = q % NB - 1, j = (q / NB) % NB - 1, k = q / (NB * NB) - 1;
+  bool in = i >= 1 && i <= NC && j >= 1 && j <= NC && k >= 1 && k <= NC;
+  if (in)
+    dst[t] = src[bv * NINT + ((k - 1) * NC + (j - 1)) * NC + (i - 1)];
+  else if (zero_ghosts)
+    dst[t] = 0.0;
+}
+
+__global__ void k_gather(int nv, int nblk, const double *src, double *dst) {
+  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t total = (size_t)nblk * nv * NINT;
+  if (t >= total) return;
+  int q = (int)(t % NINT);
+  size_t bv = t / NINT;
+  int i = q % NC + 1, j = (q / NC) % NC + 1, k = q / (NC * NC) + 1;
+  dst[t] = src[bv * NCELL + cid(i, j, k)];
+}
+
+int to_fortran_or_back(Ctx *c, const double *in, double *out, bool to_blocks, int nbmax, const int *lb) {
+  // Fortran U(nbmax, nv, -1:18, -1:18, -1:18): slot fastest
+  int nv = c->cfg.nv;
+  for (int s = 0; s < nbmax; s++) {
+    auto it = c->index.find(lb[s]);
+    if (lb[s] == -1 || it == c->index.end()) continue;
+    int b = it->second;
+    for (int v = 0; v < nv; v++)
+      for (int k = 1; k <= NC; k++)
+        for (int j = 1; j <= NC; j++)
+          for (int i = 1; i <= NC; i++) {
+            size_t fo = (size_t)s + (size_t)nbmax * (v + (size_t)nv * ((i + 1) + NB * ((j + 1) + (size_t)NB * (k + 1))));
+            size_t bo = (((size_t)b * nv + v) * NINT) + ((k - 1) * NC + (j - 1)) * NC + (i - 1);
+            if (to_blocks)
+              out[bo] = in[fo];
+            else
+              out[fo] = in[bo];
+          }
+  }
+  return WX_OK;
+}
+
+int launch_check(const char *what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(WX_EHIP, std::string(what) + ": " + hipGetErrorString(e));
+  return WX_OK;
+}
+
+// ------------------------- plain-form sequences ----------------------------
+int plain_timestep_raw(Ctx *c, double *dt) {
+  HIPCHK(hipMemsetAsync(c->d_scal, 0xFF, sizeof(unsigned long long), c->st));
+  plain_timestep(c->dc, c->dP, c->d_scal, c->st);
+  unsigned long long bits = 0;
+  HIPCHK(hipMemcpyAsync(&bits, c->d_scal, sizeof(bits), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  double m;
+  std::memcpy(&m, &bits, sizeof(m));
+  // the reference's dt_loc = dt_hydro*CFL (or min(dt_hydro, dt_cool)*CFL,
+  // dt_cool = 1e30 unused)
+  if (!c->cfg.cooling)
+    *dt = m * c->cfg.cfl;
+  else
+    *dt = std::fmin(m, 1.0e30) * c->cfg.cfl;
+  return WX_OK;
+}
+
+int plain_godunov(Ctx *c, int order, double dt) {
+  double dtp = order == 2 ? dt / 2.0 : dt;
+  plain_boundary(c->dc, c->dU, 1, c->st);
+  plain_prims(c->dc, c->dU, c->dP, WX_CELLS_GHOST, nullptr, c->st);
+  plain_fluxes(c->dc, c->dP, c->dF, 1, c->st);
+  plain_upwind(c->dc, c->dU, c->dP, c->dF, c->dUP, dtp, c->st);
+  if (order == 2) {
+    plain_boundary(c->dc, c->dUP, 2, c->st);
+    plain_prims(c->dc, c->dUP, c->dP, WX_CELLS_ALL, nullptr, c->st);
+    plain_fluxes(c->dc, c->dP, c->dF, 2, c->st);
+    plain_upwind(c->dc, c->dU, c->dP, c->dF, c->dUP, dt, c->st);
+  }
+  return launch_check("godunov");
+}
+
+int update_prims(Ctx *c, long *bad) {
+  HIPCHK(hipMemsetAsync(c->d_scal + 1, 0, sizeof(unsigned long long), c->st));
+  plain_prims(c->dc, c->dU, c->dP, WX_CELLS_ALL, c->d_scal + 1, c->st);
+  if (bad) {
+    unsigned long long n = 0;
+    HIPCHK(hipMemcpyAsync(&n, c->d_scal + 1, sizeof(n), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    *bad = (long)n;
+  }
+  return launch_check("prims");
+}
+
+Ctx *as_ctx(void *p) { return static_cast<Ctx *>(p); }
+
+int need_state(Ctx *c) {
+  if (!c) return fail(WX_EINVAL, "null context");
+  if (!c->dU) return fail(WX_ESTATE, "no mesh: call wx_set_mesh first");
+  if (!c->have_state) return fail(WX_ESTATE, "no state: call wx_upload_state first");
+  return WX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *wx_last_error(void) { return g_err.c_str(); }
+const char *wx_version(void) { return "walicxe3d-amd 0.1 (gfx950)"; }
+
+int wx_init(const wx_config *cfg, void **out) {
+  if (!cfg || !out) return fail(WX_EINVAL, "null argument");
+  if (cfg->nv != 5 && cfg->nv != 8) return fail(WX_EINVAL, "nv must be 5 or 8");
+  if (cfg->solver < 2 || cfg->solver > 6) return fail(WX_EINVAL, "unsupported solver");
+  if ((cfg->solver == 5 || cfg->solver == 6) && cfg->nv != 8) return fail(WX_EINVAL, "HLLE/HLLD need nv = 8");
+  if (cfg->limiter < -1 || cfg->limiter > 6) return fail(WX_EINVAL, "unknown limiter");
+  if (cfg->cooling && (cfg->ntab < 2 || cfg->ntab > MAXTAB || !cfg->tab_logT || !cfg->tab_loglam))
+    return fail(WX_EINVAL, "cooling needs a table of 2..512 points");
+  if (cfg->nwind < 0 || cfg->nwind > MAXWIND || (cfg->nwind && !cfg->winds))
+    return fail(WX_EINVAL, "up to 8 wind sources");
+  if (cfg->form != WX_FORM_PLAIN_FP64 && cfg->form != WX_FORM_FUSED_FP32) return fail(WX_EINVAL, "unknown form");
+  if (cfg->form == WX_FORM_FUSED_FP32 && cfg->solver == SOLVER_HLL1)
+    return fail(WX_EINVAL, "the fused form runs the two-stage scheme (SOLVER_HLL1 needs the plain form)");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(WX_EHIP, "no HIP device");
+  Ctx *c = new Ctx();
+  c->cfg = *cfg;
+  if (cfg->cooling) {
+    c->tabT.assign(cfg->tab_logT, cfg->tab_logT + cfg->ntab);
+    c->tabC.assign(cfg->tab_loglam, cfg->tab_loglam + cfg->ntab);
+    for (int i = 1; i < cfg->ntab; i++)
+      if (!(c->tabT[i] > c->tabT[i - 1])) {
+        delete c;
+        return fail(WX_EINVAL, "cooling table temperatures must increase");
+      }
+  }
+  if (cfg->nwind) c->winds.assign(cfg->winds, cfg->winds + 8 * cfg->nwind);
+  c->cfg.tab_logT = c->tabT.empty() ? nullptr : c->tabT.data();
+  c->cfg.tab_loglam = c->tabC.empty() ? nullptr : c->tabC.data();
+  c->cfg.winds = c->winds.empty() ? nullptr : c->winds.data();
+  hipGetDevice(&c->device);
+  if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return fail(WX_EHIP, "stream creation failed");
+  }
+  if (hipMalloc(&c->d_scal, 4 * sizeof(unsigned long long)) != hipSuccess) {
+    hipStreamDestroy(c->st);
+    delete c;
+    return fail(WX_ENOMEM, "device allocation failed");
+  }
+  if (cfg->cooling) {
+    hipMalloc(&c->d_tabT, sizeof(double) * cfg->ntab);
+    hipMalloc(&c->d_tabC, sizeof(double) * cfg->ntab);
+    hipMemcpy(c->d_tabT, c->tabT.data(), sizeof(double) * cfg->ntab, hipMemcpyHostToDevice);
+    hipMemcpy(c->d_tabC, c->tabC.data(), sizeof(double) * cfg->ntab, hipMemcpyHostToDevice);
+  }
+  fill_devcfg(c, nullptr);
+  *out = c;
+  return WX_OK;
+}
+
+int wx_finalize(void *ctx) {
+  Ctx *c = as_ctx(ctx);
+  if (!c) return WX_OK;
+  hipStreamSynchronize(c->st);
+  free_mesh(c);
+  hipFree(c->d_scal);
+  hipFree(c->d_tabT);
+  hipFree(c->d_tabC);
+  hipStreamDestroy(c->st);
+  delete c;
+  return WX_OK;
+}
+
+int wx_set_mesh(void *ctx, const int nroot[3], int maxlev, const double *dxyz, int nblk, const int *bids) {
+  Ctx *c = as_ctx(ctx);
+  if (!c || !nroot || !dxyz || !bids) return fail(WX_EINVAL, "null argument");
+  if (maxlev < 1 || maxlev > MAXLEV || nblk < 1) return fail(WX_EINVAL, "bad mesh size");
+  HIPCHK(hipStreamSynchronize(c->st));
+  free_mesh(c);
+  c->nblk = nblk;
+  c->maxlev = maxlev;
+  for (int d = 0; d < 3; d++) c->nroot[d] = nroot[d];
+  int rc = build_topology(c, bids);
+  if (rc != WX_OK) return rc;
+  size_t n = (size_t)nblk * c->cfg.nv * NCELL;
+  HIPCHK(hipMalloc(&c->d_info, sizeof(BlockInfo) * nblk));
+  HIPCHK(hipMemcpy(c->d_info, c->info.data(), sizeof(BlockInfo) * nblk, hipMemcpyHostToDevice));
+  HIPCHK(hipMalloc(&c->dU, n * sizeof(double)));
+  HIPCHK(hipMalloc(&c->dP, n * sizeof(double)));
+  HIPCHK(hipMemset(c->dP, 0, n * sizeof(double)));
+  if (c->cfg.form == WX_FORM_PLAIN_FP64) {
+    HIPCHK(hipMalloc(&c->dUP, n * sizeof(double)));
+    HIPCHK(hipMemset(c->dUP, 0, n * sizeof(double)));
+    HIPCHK(hipMalloc(&c->dF, plain_flux_elems(nblk, c->cfg.nv) * sizeof(double)));
+  }
+  fill_devcfg(c, dxyz);
+  if (c->cfg.form == WX_FORM_FUSED_FP32) {
+    int frc = fused_alloc(c->fused, c->dc, c->info, c->st);
+    if (frc != 0) return fail(WX_ENOMEM, "fused-form allocation failed");
+  }
+  return WX_OK;
+}
+
+int wx_upload_state(void *ctx, const double *u, int layout, int nbmax, const int *lb) {
+  Ctx *c = as_ctx(ctx);
+  if (!c || !u) return fail(WX_EINVAL, "null argument");
+  if (!c->dU) return fail(WX_ESTATE, "no mesh");
+  int nv = c->cfg.nv;
+  size_t ni = (size_t)c->nblk * nv * NINT;
+  std::vector<double> tmp;
+  const double *src = u;
+  if (layout == WX_LAYOUT_FORTRAN) {
+    if (!lb || nbmax < 1) return fail(WX_EINVAL, "Fortran layout needs nbmax and localBlocks");
+    tmp.assign(ni, 0.0);
+    to_fortran_or_back(c, u, tmp.data(), true, nbmax, lb);
+    src = tmp.data();
+  } else if (layout != WX_LAYOUT_BLOCKS) {
+    return fail(WX_EINVAL, "unknown layout");
+  }
+  double *d_in = nullptr;
+  HIPCHK(hipMalloc(&d_in, ni * sizeof(double)));
+  HIPCHK(hipMemcpyAsync(d_in, src, ni * sizeof(double), hipMemcpyHostToDevice, c->st));
+  size_t total = (size_t)c->nblk * nv * NCELL;
+  hipLaunchKernelGGL(k_scatter, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, c->st, nv, c->nblk, d_in, c->dU, 1);
+  int rc = launch_check("upload");
+  HIPCHK(hipStreamSynchronize(c->st));
+  hipFree(d_in);
+  if (rc != WX_OK) return rc;
+  c->have_state = true;
+  if (c->cfg.form == WX_FORM_FUSED_FP32) return fused_upload(c->fused, c->dc, c->dU, c->st) ? WX_EHIP : WX_OK;
+  long bad = 0;
+  rc = update_prims(c, &bad);
+  if (rc != WX_OK) return rc;
+  return (int)bad;
+}
+
+int wx_download_state(void *ctx, double *u, double *prim, int layout, int nbmax, const int *lb) {
+  Ctx *c = as_ctx(ctx);
+  int rc = need_state(c);
+  if (rc != WX_OK) return rc;
+  int nv = c->cfg.nv;
+  size_t ni = (size_t)c->nblk * nv * NINT;
+  if (c->cfg.form == WX_FORM_FUSED_FP32) {
+    if (fused_download(c->fused, c->dc, c->dU, c->dP, c->st)) return fail(WX_EHIP, "fused download failed");
+  }
+  double *d_out = nullptr;
+  HIPCHK(hipMalloc(&d_out, ni * sizeof(double)));
+  std::vector<double> tmp(layout == WX_LAYOUT_FORTRAN ? ni : 0);
+  for (int w = 0; w < 2; w++) {
+    double *dst = w == 0 ? u : prim;
+    if (!dst) continue;
+    hipLaunchKernelGGL(k_gather, dim3((unsigned)((ni + 255) / 256)), dim3(256), 0, c->st, nv, c->nblk,
+                       w == 0 ? c->dU : c->dP, d_out);
+    double *h = layout == WX_LAYOUT_FORTRAN ? tmp.data() : dst;
+    HIPCHK(hipMemcpyAsync(h, d_out, ni * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    if (layout == WX_LAYOUT_FORTRAN) to_fortran_or_back(c, tmp.data(), dst, false, nbmax, lb);
+  }
+  hipFree(d_out);
+  return launch_check("download");
+}
+
+int wx_get_timestep(void *ctx, double *dt_raw) {
+  Ctx *c = as_ctx(ctx);
+  int rc = need_state(c);
+  if (rc != WX_OK) return rc;
+  if (!dt_raw) return fail(WX_EINVAL, "null dt");
+  if (c->cfg.form == WX_FORM_FUSED_FP32) return fused_timestep(c->fused, c->dc, c->st, dt_raw) ? WX_EHIP : WX_OK;
+  return plain_timestep_raw(c, dt_raw);
+}
+
+int wx_godunov(void *ctx, int order, double dt) {
+  Ctx *c = as_ctx(ctx);
+  int rc = need_state(c);
+  if (rc != WX_OK) return rc;
+  if (order != 1 && order != 2) return fail(WX_EINVAL, "order must be 1 or 2");
+  if (c->cfg.form == WX_FORM_FUSED_FP32) return fail(WX_EINVAL, "wx_godunov is a plain-form entry point");
+  return plain_godunov(c, order, dt);
+}
+
+int wx_viscous_step(void *ctx) {
+  Ctx *c = as_ctx(ctx);
+  int rc = need_state(c);
+  if (rc != WX_OK) return rc;
+  if (c->cfg.form == WX_FORM_FUSED_FP32) return fail(WX_EINVAL, "plain-form entry point");
+  plain_viscous(c->dc, c->dUP, c->dU, c->st);
+  return launch_check("viscous");
+}
+
+int wx_update_prims(void *ctx) {
+  Ctx *c = as_ctx(ctx);
+  int rc = need_state(c);
+  if (rc != WX_OK) return rc;
+  if (c->cfg.form == WX_FORM_FUSED_FP32) return fail(WX_EINVAL, "plain-form entry point");
+  long bad = 0;
+  rc = update_prims(c, &bad);
+  return rc != WX_OK ? rc : (int)bad;
+}
+
+int wx_cooling(void *ctx, double dt) {
+  Ctx *c = as_ctx(ctx);
+  int rc = need_state(c);
+  if (rc != WX_OK) return rc;
+  if (c->cfg.form == WX_FORM_FUSED_FP32) return fail(WX_EINVAL, "plain-form entry point");
+  if (!c->cfg.cooling) return WX_OK;
+  plain_cooling(c->dc, c->dU, c->dP, dt, c->st);
+  return launch_check("cooling");
+}
+
+int wx_step(void *ctx, int *it, double *dt, double *time, int *nextout, double dtout, double t_sc) {
+  Ctx *c = as_ctx(ctx);
+  int rc = need_state(c);
+  if (rc != WX_OK) return rc;
+  if (!it || !dt || !time || !nextout) return fail(WX_EINVAL, "null argument");
+  if (c->cfg.form == WX_FORM_FUSED_FP32) return wx_run(ctx, 1, it, dt, time, nextout, dtout, t_sc);
+  *it += 1;
+  double dtl;
+  rc = plain_timestep_raw(c, &dtl);
+  if (rc != WX_OK) return rc;
+  // start-up ramp and output clamp (source/hydro_solver.f90:191-207)
+  if (*it <= 10) dtl = std::ldexp(1.0, -(11 - *it)) * dtl;
+  double tout = (double)(*nextout) * dtout / t_sc;
+  bool dump = false;
+  if (*time + dtl >= tout) {
+    dtl = tout - *time;
+    dump = true;
+  }
+  *dt = dtl;
+  rc = plain_godunov(c, c->cfg.solver == SOLVER_HLL1 ? 1 : 2, dtl);
+  if (rc != WX_OK) return rc;
+  *time = *time + dtl;
+  plain_viscous(c->dc, c->dUP, c->dU, c->st);
+  long bad = 0;
+  rc = update_prims(c, &bad);
+  if (rc != WX_OK) return rc;
+  if (c->cfg.cooling) plain_cooling(c->dc, c->dU, c->dP, dtl, c->st);
+  if (dump) *nextout += 1;
+  rc = launch_check("step");
+  return rc != WX_OK ? rc : (int)bad;
+}
+
+int wx_run(void *ctx, int nsteps, int *it, double *dt, double *time, int *nextout, double dtout, double t_sc) {
+  Ctx *c = as_ctx(ctx);
+  int rc = need_state(c);
+  if (rc != WX_OK) return rc;
+  if (!it || !dt || !time || !nextout || nsteps < 0) return fail(WX_EINVAL, "bad argument");
+  if (c->cfg.form != WX_FORM_FUSED_FP32) {
+    for (int s = 0; s < nsteps; s++) {
+      rc = wx_step(ctx, it, dt, time, nextout, dtout, t_sc);
+      if (rc < 0) return rc;
+    }
+    return WX_OK;
+  }
+  int frc = fused_run(c->fused, c->dc, c->st, nsteps, it, dt, time, nextout, dtout, t_sc);
+  if (frc) return fail(WX_EHIP, std::string("fused run failed: ") + fused_error());
+  return WX_OK;
+}
+
+int wx_boundary(void *ctx, int depth, int which) {
+  Ctx *c = as_ctx(ctx);
+  int rc = need_state(c);
+  if (rc != WX_OK) return rc;
+  if (c->cfg.form != WX_FORM_PLAIN_FP64) return fail(WX_EINVAL, "plain-form entry point");
+  if (depth != 1 && depth != 2) return fail(WX_EINVAL, "depth must be 1 or 2");
+  plain_boundary(c->dc, which == WX_UP ? c->dUP : c->dU, depth, c->st);
+  return launch_check("boundary");
+}
+
+int wx_calc_prims(void *ctx, int which, int cells) {
+  Ctx *c = as_ctx(ctx);
+  int rc = need_state(c);
+  if (rc != WX_OK) return rc;
+  if (c->cfg.form != WX_FORM_PLAIN_FP64) return fail(WX_EINVAL, "plain-form entry point");
+  if (cells < 0 || cells > 2) return fail(WX_EINVAL, "bad cell range");
+  plain_prims(c->dc, which == WX_UP ? c->dUP : c->dU, c->dP, cells, nullptr, c->st);
+  return launch_check("calc_prims");
+}
+
+int wx_fluxes_update(void *ctx, int order, double dtp) {
+  Ctx *c = as_ctx(ctx);
+  int rc = need_state(c);
+  if (rc != WX_OK) return rc;
+  if (c->cfg.form != WX_FORM_PLAIN_FP64) return fail(WX_EINVAL, "plain-form entry point");
+  plain_fluxes(c->dc, c->dP, c->dF, order, c->st);
+  plain_upwind(c->dc, c->dU, c->dP, c->dF, c->dUP, dtp, c->st);
+  return launch_check("fluxes_update");
+}
+
+int wx_copy_raw(void *ctx, int which, double *out) {
+  Ctx *c = as_ctx(ctx);
+  int rc = need_state(c);
+  if (rc != WX_OK) return rc;
+  if (c->cfg.form != WX_FORM_PLAIN_FP64) return fail(WX_EINVAL, "plain-form entry point");
+  const double *src = which == 0 ? c->dU : which == 1 ? c->dUP : c->dP;
+  size_t n = (size_t)c->nblk * c->cfg.nv * NCELL;
+  HIPCHK(hipMemcpyAsync(out, src, n * sizeof(double), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  return WX_OK;
+}
+
+int wx_set_tuning(void *ctx, int kind, int value) {
+  Ctx *c = as_ctx(ctx);
+  if (!c) return fail(WX_EINVAL, "null context");
+  return fused_set_tuning(c->fused, kind, value) ? fail(WX_EINVAL, "unknown tuning option") : WX_OK;
+}
+
+int wx_time_stages(void *ctx, int reps) {
+  Ctx *c = as_ctx(ctx);
+  int rc = need_state(c);
+  if (rc != WX_OK) return rc;
+  if (c->cfg.form != WX_FORM_FUSED_FP32) return fail(WX_EINVAL, "fused-form entry point");
+  if (reps < 1) return fail(WX_EINVAL, "reps must be positive");
+  if (fused_time_stages(c->fused, c->dc, c->st, reps)) return fail(WX_EHIP, fused_error());
+  return WX_OK;
+}
+
+int wx_last_timing(void *ctx, double *s1, double *s2, double *tot) {
+  Ctx *c = as_ctx(ctx);
+  if (!c) return fail(WX_EINVAL, "null context");
+  fused_last_timing(c->fused, s1, s2, tot);
+  return WX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,216 @@
+// wx_device.h - device-side configuration and the routines shared by the
+// plain and fused forms: ghost-cell source selection (AMR-aware face
+// exchange), wind sources, cooling.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "wx_common.h"
+#include "wx_physics.h"
+
+namespace wx {
+
+constexpr double AMU = 1.660538782e-24;   // source/constants.f90:36
+constexpr double KB = 1.380650400e-16;    // source/constants.f90:37
+constexpr double PI_REF = 3.14159265358979;  // source/constants.f90:46
+constexpr int MAXLEV = 16;
+constexpr int MAXTAB = 512;
+constexpr int MAXWIND = 8;
+
+struct DevCfg {
+  Phys<double> pd;
+  Phys<float> pf;
+  int nv, eight_wave, cooling, eos, nwind, ntab, maxlev, nblk;
+  int bc[6];
+  double visc_eta, cfl, cooling_limit, ion_thres, d_sc, e_sc, t_sc, p_sc, l_sc, v_sc;
+  double temp_mu0;   // mu0*AMU*p_sc/d_sc/KB (calcTemp, hydro_core.f90:243)
+  double temp_mui;   // mui*AMU*p_sc/d_sc/KB
+  double mui_amu;    // mui*AMU (cooling.f90:340)
+  double dx[3][MAXLEV];
+  const BlockInfo *info;
+  const double *tabT;   // device, ntab
+  const double *tabC;   // device, ntab
+  double winds[MAXWIND][8];
+};
+
+// ---------------------------------------------------------------------------
+// Face exchange: where the value of a face ghost cell comes from.
+// Ghost layer g (1 adjacent, 2 outer) of face f of block b at transverse
+// position (a1, a2) in [1,16]^2; axes: n = f/2, (t1, t2) the two others in
+// increasing order.  Follows normalBoundary (source/boundary.f90:328-668):
+// same-level copy, coarse->fine injection, fine->coarse 2x2x2 average in the
+// reference's summation order, free-flow (copy at low faces, mirror at high
+// faces) and reflective (mirror, normal momentum negated) domain faces.
+// ---------------------------------------------------------------------------
+struct GhostSrc {
+  int kind;        // 0 copy, 1 average of 8, 2 reflective copy
+  int blk;         // source block
+  int c[3];        // source cell (copy) or lowest corner of the 2x2x2 cell group
+  int flip;        // variable negated (reflective), else -1
+};
+
+__host__ __device__ inline void face_axes(int f, int &n, int &t1, int &t2) {
+  n = f >> 1;
+  t1 = (n == 0) ? 1 : 0;
+  t2 = (n == 2) ? 1 : 2;
+}
+
+// Returns false when the ghost is left untouched (periodic face without
+// partner never happens: the host rejects such meshes).
+__host__ __device__ inline bool ghost_source(const BlockInfo *info, const int *bc, int b, int f, int g, int a1,
+                                             int a2, GhostSrc &s) {
+  const BlockInfo &B = info[b];
+  const Face &F = B.face[f];
+  int n, t1, t2;
+  face_axes(f, n, t1, t2);
+  bool low = (f & 1) == 0;
+  s.flip = -1;
+  switch (F.type) {
+    case NEIGH_SAME:
+      s.kind = 0;
+      s.blk = F.src[0];
+      s.c[n] = low ? (NC + 1 - g) : g;
+      s.c[t1] = a1;
+      s.c[t2] = a2;
+      return true;
+    case NEIGH_COARSER:
+      s.kind = 0;
+      s.blk = F.src[0];
+      s.c[n] = low ? NC : 1;
+      s.c[t1] = (a1 + 1) / 2 + B.sib[t1] * (NC / 2);
+      s.c[t2] = (a2 + 1) / 2 + B.sib[t2] * (NC / 2);
+      return true;
+    case NEIGH_FINER: {
+      int q1 = a1 > NC / 2, q2 = a2 > NC / 2;
+      s.kind = 1;
+      s.blk = F.src[q1 + 2 * q2];
+      s.c[n] = low ? (NC + 1 - 2 * g) : (2 * g - 1);
+      s.c[t1] = 2 * (a1 - q1 * (NC / 2)) - 1;
+      s.c[t2] = 2 * (a2 - q2 * (NC / 2)) - 1;
+      return true;
+    }
+    default: {  // domain boundary
+      int t = bc[f];
+      s.blk = b;
+      s.c[t1] = a1;
+      s.c[t2] = a2;
+      if (t == BC_REFLECTIVE) {
+        s.kind = 2;
+        s.c[n] = low ? g : (NC + 1 - g);
+        s.flip = 1 + n;
+        return true;
+      }
+      if (t == BC_FREE_FLOW) {
+        s.kind = 0;
+        s.c[n] = low ? 1 : (NC + 1 - g);
+        return true;
+      }
+      return false;
+    }
+  }
+}
+
+// Value of variable v of a face ghost from its source, reading an array
+// with ghosts ([blk][nv][20^3]) of element type T, computing in R.
+template <typename R, typename T>
+__device__ inline R ghost_value(const T *A, int nv, const GhostSrc &s, int v) {
+  const T *S = A + ((size_t)s.blk * nv + v) * NCELL;
+  if (s.kind == 1) {
+    R acc = R(0);
+    for (int dx = 0; dx < 2; dx++)
+      for (int dy = 0; dy < 2; dy++)
+        for (int dz = 0; dz < 2; dz++)
+          acc = acc + R(S[cid(s.c[0] + dx, s.c[1] + dy, s.c[2] + dz)]) / R(8.0);
+    return acc;
+  }
+  R val = R(S[cid(s.c[0], s.c[1], s.c[2])]);
+  if (s.kind == 2 && v == s.flip) val = R(-1.0) * val;
+  return val;
+}
+
+// ---------------------------------------------------------------------------
+// Spherical wind sources (imposeSphericalWind, source/winds.f90:123-240):
+// cells whose centre lies within the source radius get the wind state.
+// cellPos (source/amr.f90:1037-1065) gives the centre.
+// ---------------------------------------------------------------------------
+__device__ inline bool wind_state(const DevCfg &c, int b, int i, int j, int k, double *u) {
+  const BlockInfo &B = c.info[b];
+  int l = B.level - 1;
+  double x = ((B.coords[0] - 1) * NC + (i - 1) + 0.5) * c.dx[0][l];
+  double y = ((B.coords[1] - 1) * NC + (j - 1) + 0.5) * c.dx[1][l];
+  double z = ((B.coords[2] - 1) * NC + (k - 1) + 0.5) * c.dx[2][l];
+  x = x * c.l_sc;
+  y = y * c.l_sc;
+  z = z * c.l_sc;
+  bool hit = false;
+  for (int w = 0; w < c.nwind; w++) {
+    const double *wp = c.winds[w];
+    double dist = sqrt((x - wp[0]) * (x - wp[0]) + (y - wp[1]) * (y - wp[1]) + (z - wp[2]) * (z - wp[2]));
+    if (dist < wp[3]) {
+      double dens = wp[4] / wp[5] / dist / dist / (4.0 * PI_REF);
+      double vx = wp[5] * (x - wp[0]) / dist + 0.0;
+      double vy = wp[5] * (y - wp[1]) / dist + 0.0;
+      double vz = wp[5] * (z - wp[2]) / dist + 0.0;
+      double pres = dens / (wp[7] * AMU) * KB * wp[6];
+      double p[MAXV] = {0, 0, 0, 0, 0, 0, 0, 0};
+      p[0] = dens / c.d_sc;
+      p[1] = vx / c.v_sc;
+      p[2] = vy / c.v_sc;
+      p[3] = vz / c.v_sc;
+      p[4] = pres / c.p_sc;
+      prim2flow(c.pd, p, u);
+      hit = true;
+    }
+  }
+  return hit;
+}
+
+// ---------------------------------------------------------------------------
+// Cooling (apply_cooling, source/cooling.f90:285-371; find_coef :378-419;
+// calcTemp, source/hydro_core.f90:231-270).  Updates p and returns E.
+// ---------------------------------------------------------------------------
+__device__ inline double find_coef(const DevCfg &c, double logT) {
+  const double *T = c.tabT, *C = c.tabC;
+  int n = c.ntab;
+  if (logT < T[0]) return 0.0;
+  if (logT >= T[n - 1]) return pow(10.0, C[n - 1] + 0.5 * (logT - T[n - 1]));
+  // first node above logT: binary search gives the same node as the
+  // reference's linear scan of a strictly increasing table
+  int lo = 0, hi = n - 1;
+  while (hi - lo > 1) {
+    int mid = (lo + hi) >> 1;
+    if (T[mid] > logT) hi = mid; else lo = mid;
+  }
+  double T0 = T[hi - 1], C0 = C[hi - 1], T1 = T[hi], C1 = C[hi];
+  return pow(10.0, (C1 - C0) / (T1 - T0) * (logT - T0) + C0);
+}
+
+__device__ inline double calc_temp(const DevCfg &c, double rho, double p) {
+  double t = p / rho * c.temp_mu0;
+  if (c.eos == 2 && t > c.ion_thres) t = p / rho * c.temp_mui;
+  return t;
+}
+
+// One cell; prim holds (rho, u, v, w, p).  Returns true when the cell
+// changed, with new p in prim[4] and new total energy in *e.
+__device__ inline bool cool_cell(const DevCfg &c, double dt, double *prim, double *e) {
+  double temp = calc_temp(c, prim[0], prim[4]);
+  double logT = log10(temp);
+  if (!(logT >= c.tabT[0])) return false;
+  double ETH = c.pd.cv * prim[4];
+  double vel2 = prim[1] * prim[1] + prim[2] * prim[2] + prim[3] * prim[3];
+  double EK = 0.5 * prim[0] * vel2;
+  double aloss = find_coef(c, logT);
+  double numdens = prim[0] * c.d_sc / c.mui_amu;
+  double ce = aloss * (numdens * numdens) / (ETH * c.e_sc);
+  double cool_factor = exp(-ce * (dt * c.t_sc));
+  if (cool_factor < 1.0 - c.cooling_limit) cool_factor = 1.0 - c.cooling_limit;
+  double new_temp = temp * cool_factor;
+  if (new_temp < 10.0) cool_factor = 10.0 / temp;
+  prim[4] = prim[4] * cool_factor;
+  ETH = c.pd.cv * prim[4];
+  *e = EK + ETH;
+  return true;
+}
+
+}  // namespace wx

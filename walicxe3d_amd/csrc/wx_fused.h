@@ -1,0 +1,55 @@
+// wx_fused.h - the fused FP32 form: one kernel per stage over every block of
+// every level, halos gathered from neighbour interiors on the fly, FP64
+// conserved accumulation, device-side dt, HIP graph per mesh.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "wx_device.h"
+
+namespace wx {
+
+// Device-resident step scalars (the host never reads them between steps).
+struct StepState {
+  double dt;          // dt of the current step
+  double time;        // simulation time at the start of the step
+  double dtout;       // output interval (parameters dtout)
+  double t_sc;        // time unit (parameters t_sc)
+  unsigned long long dtmin_bits;  // running min of dx/(|v|+c) over cells
+  int it;             // iteration counter
+  int nextout;        // next output number
+  int dump;           // the clamp fired this step
+  int pad;
+};
+
+struct FusedState {
+  double *U = nullptr;       // [blk][nv][4096] FP64 conserved state
+  float *UP = nullptr;       // [blk][nv][4096] FP32 half-step state
+  int *order = nullptr;      // block processing order (device)
+  StepState *step = nullptr; // device
+  int nblk = 0, nv = 0;
+  int grid = 0;              // stage-kernel grid size (0 = one workgroup per block)
+  int xcd_map = 1;           // XCD-aware block order
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  double t_s1 = 0, t_s2 = 0, t_tot = 0;
+  int timing_runs = 0;
+  std::vector<int> host_order;
+};
+
+int fused_alloc(FusedState &f, const DevCfg &c, const std::vector<BlockInfo> &info, hipStream_t st);
+void fused_free(FusedState &f);
+int fused_upload(FusedState &f, const DevCfg &c, const double *dU_ghosted, hipStream_t st);
+int fused_download(FusedState &f, const DevCfg &c, double *dU_ghosted, double *dP_ghosted, hipStream_t st);
+int fused_timestep(FusedState &f, const DevCfg &c, hipStream_t st, double *dt_raw);
+int fused_run(FusedState &f, const DevCfg &c, hipStream_t st, int nsteps, int *it, double *dt, double *time,
+              int *nextout, double dtout, double t_sc);
+int fused_time_stages(FusedState &f, const DevCfg &c, hipStream_t st, int reps);
+int fused_set_tuning(FusedState &f, int kind, int value);
+void fused_last_timing(const FusedState &f, double *s1, double *s2, double *tot);
+const char *fused_error();
+
+}  // namespace wx

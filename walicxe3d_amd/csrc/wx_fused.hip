@@ -1,0 +1,699 @@
+// wx_fused.hip - the fused FP32 form of the Godunov update.
+//
+// One stage = one kernel launch over all blocks of all levels.  A workgroup
+// of 256 threads owns a block; thread (ti, tj) owns the column (i, j) and
+// marches in k.  Per k-plane the primitives of the plane and its x/y face
+// halo sit in LDS (x and y fluxes), the column's primitives of the planes
+// around k sit in registers (z fluxes, Powell term).  Face halos are read
+// straight from the neighbour blocks' interiors with the AMR exchange rules
+// of wx_device.h - no ghost cells are stored in HBM.
+//
+// Precision: bulk flux/reconstruction arithmetic in FP32; conserved state U
+// and its accumulation U + dt*dF in FP64; stage-1 pressure recovery from the
+// FP64 state in FP64; the stage-2 epilogue (viscosity, primitives, cooling,
+// dt candidate) in FP64 on U^{n+1}; dt reduced as an exact min.
+//
+// Stage 1 (first order, dt/2): U^n (FP64) -> UP (FP32).
+// Stage 2 (second order, dt): UP, U^n -> U^{n+1} (in place), with the
+// artificial viscosity (its face neighbours are the half-step ghosts, as in
+// the reference), updatePrims, cooling and the next dt fused in.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+
+#include "wx_fused.h"
+
+namespace wx {
+
+static thread_local std::string f_err;
+const char *fused_error() { return f_err.c_str(); }
+
+__device__ __forceinline__ int lin(int i, int j, int k) { return ((k - 1) * NC + (j - 1)) * NC + (i - 1); }
+
+struct FusedArgs {
+  DevCfg c;
+  const double *U;   // stage 1 input (FP64) / stage 2 base
+  double *Uout;      // stage 2 output (in place)
+  const float *UPin; // stage 2 input
+  float *UPout;      // stage 1 output
+  const int *order;
+  StepState *step;
+  int nblk;
+};
+
+// conserved state of cell (i,j,k) of block b: interior or face ghost,
+// then the wind sources; T is the storage type of the input array
+template <typename T, int NV>
+__device__ __forceinline__ void load_cell(const FusedArgs &a, const T *W, int b, int i, int j, int k, double *u) {
+  bool in = (unsigned)(i - 1) < NC && (unsigned)(j - 1) < NC && (unsigned)(k - 1) < NC;
+  if (in) {
+    const T *Wb = W + (size_t)b * NV * NINT + lin(i, j, k);
+#pragma unroll
+    for (int v = 0; v < NV; v++) u[v] = (double)Wb[v * NINT];
+  } else {
+    int f, g, a1, a2;
+    if (i < 1) { f = LEFT; g = 1 - i; a1 = j; a2 = k; }
+    else if (i > NC) { f = RIGHT; g = i - NC; a1 = j; a2 = k; }
+    else if (j < 1) { f = FRONT; g = 1 - j; a1 = i; a2 = k; }
+    else if (j > NC) { f = BACK; g = j - NC; a1 = i; a2 = k; }
+    else if (k < 1) { f = BOTTOM; g = 1 - k; a1 = i; a2 = j; }
+    else { f = TOP; g = k - NC; a1 = i; a2 = j; }
+    GhostSrc s;
+    ghost_source(a.c.info, a.c.bc, b, f, g, a1, a2, s);
+    const T *S = W + (size_t)s.blk * NV * NINT;
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+      const T *Sv = S + v * NINT;
+      double val;
+      if (s.kind == 1) {
+        double acc = 0.0;
+        for (int dx = 0; dx < 2; dx++)
+          for (int dy = 0; dy < 2; dy++)
+            for (int dz = 0; dz < 2; dz++) acc = acc + (double)Sv[lin(s.c[0] + dx, s.c[1] + dy, s.c[2] + dz)] / 8.0;
+        val = acc;
+      } else {
+        val = (double)Sv[lin(s.c[0], s.c[1], s.c[2])];
+        if (s.kind == 2 && v == s.flip) val = -val;
+      }
+      u[v] = val;
+    }
+  }
+  if (a.c.nwind > 0) {
+    double w[MAXV];
+    if (wind_state(a.c, b, i, j, k, w)) {
+#pragma unroll
+      for (int v = 0; v < NV; v++) u[v] = w[v];
+    }
+  }
+}
+
+template <int NV>
+__device__ __forceinline__ void prims_f(const DevCfg &c, const double *u, float *p, bool fp64) {
+  if (fp64) {
+    double pd[MAXV];
+    flow2prim(c.pd, u, pd);
+#pragma unroll
+    for (int v = 0; v < NV; v++) p[v] = (float)pd[v];
+  } else {
+    float uf[MAXV];
+#pragma unroll
+    for (int v = 0; v < NV; v++) uf[v] = (float)u[v];
+    flow2prim(c.pf, uf, p);
+  }
+}
+
+// one interface flux from four x-aligned states (order 2) or two (order 1);
+// dim selects the rotation
+template <int NV, int ORD>
+__device__ __forceinline__ void iface_flux(const Phys<float> &P, int dim, float *pll, float *pl, float *pr,
+                                           float *prr, float *ff) {
+  swapdim(NV, dim, pl);
+  swapdim(NV, dim, pr);
+  if (ORD == 2) {
+    swapdim(NV, dim, pll);
+    swapdim(NV, dim, prr);
+#pragma unroll
+    for (int v = 0; v < NV; v++) limit1(pll[v], pl[v], pr[v], prr[v], P.limiter);
+  }
+  riemann(P, pl, pr, ff);
+  swapdim(NV, dim, ff);
+}
+
+template <int NV>
+__device__ __forceinline__ float dt_cell(const DevCfg &c, const double *pd, float dx, float dy, float dz) {
+  float p[MAXV];
+#pragma unroll
+  for (int v = 0; v < NV; v++) p[v] = (float)pd[v];
+  float r;
+  if (NV == 8 && c.pf.mhd) {
+    float cfx = cfast1(c.pf, p[4], p[0], p[5], p[6], p[7], p[5]);
+    float cfy = cfast1(c.pf, p[4], p[0], p[5], p[6], p[7], p[6]);
+    float cfz = cfast1(c.pf, p[4], p[0], p[5], p[6], p[7], p[7]);
+    r = fminf(fminf(dx / (fabsf(p[1]) + cfx), dy / (fabsf(p[2]) + cfy)), dz / (fabsf(p[3]) + cfz));
+  } else {
+    float cs = sound(c.pf, p);
+    r = fminf(fminf(dx / (fabsf(p[1]) + cs), dy / (fabsf(p[2]) + cs)), dz / (fabsf(p[3]) + cs));
+  }
+  return r;
+}
+
+// ---------------------------------------------------------------------------
+// Stage kernel.  S2 = second stage; VISC = artificial viscosity active.
+// ---------------------------------------------------------------------------
+template <int NV, bool S2, bool VISC>
+__global__ void __launch_bounds__(256) k_stage(FusedArgs a) {
+  constexpr int ORD = S2 ? 2 : 1;
+  constexpr int H = ORD;
+  constexpr int W = NC + 2 * H;
+  constexpr int NW = S2 ? 4 : 3;               // planes in the column window
+  __shared__ float sp[NV][W][W];               // primitives of the plane (cross region)
+  __shared__ float sfx[NV][NC][NC + 1];        // x fluxes [v][j-1][s]
+  __shared__ float sfy[NV][NC + 1][NC];        // y fluxes [v][s][i-1]
+  __shared__ float sun[(S2 && VISC) ? 2 : 1][(S2 && VISC) ? NV : 1][NC + 2][NC + 2];  // UP_new planes + ghost ring
+  __shared__ float sred[4];
+
+  const DevCfg &c = a.c;
+  const int tid = threadIdx.x;
+  const int ti = tid & 15, tj = tid >> 4;
+  const int i = ti + 1, j = tj + 1;
+  const double dt = a.step->dt;
+  const double dtp = S2 ? dt : dt / 2.0;
+  float dtmin = 3.0e38f;
+
+  for (int bi = blockIdx.x; bi < a.nblk; bi += gridDim.x) {
+    const int b = a.order[bi];
+    const int l = c.info[b].level - 1;
+    const double dxl = c.dx[0][l], dyl = c.dx[1][l], dzl = c.dx[2][l];
+    const double dtdx = dtp / dxl, dtdy = dtp / dyl, dtdz = dtp / dzl;
+    const float idx2 = (float)(1.0 / (2.0 * dxl)), idy2 = (float)(1.0 / (2.0 * dyl)), idz2 = (float)(1.0 / (2.0 * dzl));
+
+    // ---- column window of primitives: win[w] = plane (k - 1 + w) at iteration k
+    float win[NW][NV];
+    double ucol[MAXV];
+    double ughost_lo[MAXV], ughost_hi[MAXV];   // conserved UP ghosts at planes 0 and 17 (viscosity)
+    // pre-loop planes: S2: -1, 0, 1 (win[0..2] for k=0 -> shifted below); S1: 0, 1
+    {
+      const int k0 = S2 ? -1 : 0;
+      for (int w = 0; w < NW - 1; w++) {
+        int kk = k0 + w;
+        if (S2) load_cell<float, NV>(a, a.UPin, b, i, j, kk, ucol);
+        else load_cell<double, NV>(a, a.U, b, i, j, kk, ucol);
+        if (S2 && kk == 0) {
+#pragma unroll
+          for (int v = 0; v < NV; v++) ughost_lo[v] = ucol[v];
+        }
+        prims_f<NV>(c, ucol, win[w + (S2 ? 0 : 1)], !S2);
+      }
+    }
+    // z flux (0|1)
+    float fzp[NV];
+    {
+      float pll[NV], pl[NV], pr[NV], prr[NV];
+      if (S2) {
+        // need plane 2 as well
+        load_cell<float, NV>(a, a.UPin, b, i, j, 2, ucol);
+        prims_f<NV>(c, ucol, win[3], false);
+#pragma unroll
+        for (int v = 0; v < NV; v++) { pll[v] = win[0][v]; pl[v] = win[1][v]; pr[v] = win[2][v]; prr[v] = win[3][v]; }
+      } else {
+#pragma unroll
+        for (int v = 0; v < NV; v++) { pl[v] = win[1][v]; pr[v] = win[2][v]; }
+      }
+      iface_flux<NV, ORD>(c.pf, 2, pll, pl, pr, prr, fzp);
+    }
+    // after the pre-loop: S2 win = {-1, 0, 1, 2}; S1 win = {?, 0, 1}
+    double unew_prev[MAXV], unew_prev2[MAXV];   // UP_new at planes k-1, k-2 (stage 2)
+
+    for (int k = 1; k <= NC; k++) {
+      __syncthreads();   // previous plane's LDS readers are done
+      // shift the window: S2 -> {k-1, k, k+1, k+2}; S1 -> {k-1, k, k+1}
+#pragma unroll
+      for (int w = 0; w < NW - 1; w++)
+#pragma unroll
+        for (int v = 0; v < NV; v++) win[w][v] = win[w + 1][v];
+      {
+        int kn = S2 ? k + 2 : k + 1;
+        if (S2) load_cell<float, NV>(a, a.UPin, b, i, j, kn, ucol);
+        else load_cell<double, NV>(a, a.U, b, i, j, kn, ucol);
+        if (S2 && kn == NC + 1) {
+#pragma unroll
+          for (int v = 0; v < NV; v++) ughost_hi[v] = ucol[v];
+        }
+        prims_f<NV>(c, ucol, win[NW - 1], !S2);
+      }
+      // own primitives of plane k into LDS; halo threads add the face halo
+#pragma unroll
+      for (int v = 0; v < NV; v++) sp[v][j - 1 + H][i - 1 + H] = win[1][v];
+      if (tid < 4 * H * NC) {
+        int part = tid / (2 * H * NC);
+        int r = tid % (2 * H * NC);
+        int side = r / (H * NC);
+        int g = (r % (H * NC)) / NC + 1;
+        int aa = r % NC + 1;
+        int hi, hj;
+        if (part == 0) { hi = side ? NC + g : 1 - g; hj = aa; }
+        else { hj = side ? NC + g : 1 - g; hi = aa; }
+        double uh[MAXV];
+        float ph[MAXV];
+        if (S2) load_cell<float, NV>(a, a.UPin, b, hi, hj, k, uh);
+        else load_cell<double, NV>(a, a.U, b, hi, hj, k, uh);
+        prims_f<NV>(c, uh, ph, !S2);
+#pragma unroll
+        for (int v = 0; v < NV; v++) sp[v][hj - 1 + H][hi - 1 + H] = ph[v];
+        if (S2 && VISC && g == 1) {
+#pragma unroll
+          for (int v = 0; v < NV; v++) sun[k & 1][(S2 && VISC) ? v : 0][hj][hi] = (float)uh[v];
+        }
+      }
+      __syncthreads();
+      // x and y fluxes of plane k: 272 + 272 interfaces
+      for (int t = tid; t < 2 * (NC + 1) * NC; t += 256) {
+        int dim = t >= (NC + 1) * NC;
+        int r = dim ? t - (NC + 1) * NC : t;
+        int s = r % (NC + 1);     // interface (s | s+1)
+        int aa = r / (NC + 1) + 1;
+        float pll[NV], pl[NV], pr[NV], prr[NV], ff[NV];
+#pragma unroll
+        for (int v = 0; v < NV; v++) {
+          if (dim == 0) {
+            pl[v] = sp[v][aa - 1 + H][s - 1 + H];
+            pr[v] = sp[v][aa - 1 + H][s + H];
+            if (ORD == 2) { pll[v] = sp[v][aa - 1 + H][s - 2 + H]; prr[v] = sp[v][aa - 1 + H][s + 1 + H]; }
+          } else {
+            pl[v] = sp[v][s - 1 + H][aa - 1 + H];
+            pr[v] = sp[v][s + H][aa - 1 + H];
+            if (ORD == 2) { pll[v] = sp[v][s - 2 + H][aa - 1 + H]; prr[v] = sp[v][s + 1 + H][aa - 1 + H]; }
+          }
+        }
+        iface_flux<NV, ORD>(c.pf, dim, pll, pl, pr, prr, ff);
+        if (dim == 0) {
+#pragma unroll
+          for (int v = 0; v < NV; v++) sfx[v][aa - 1][s] = ff[v];
+        } else {
+#pragma unroll
+          for (int v = 0; v < NV; v++) sfy[v][s][aa - 1] = ff[v];
+        }
+      }
+      // z flux (k | k+1)
+      float fzc[NV];
+      {
+        float pll[NV], pl[NV], pr[NV], prr[NV];
+#pragma unroll
+        for (int v = 0; v < NV; v++) {
+          if (S2) { pll[v] = win[0][v]; pl[v] = win[1][v]; pr[v] = win[2][v]; prr[v] = win[3][v]; }
+          else { pl[v] = win[1][v]; pr[v] = win[2][v]; }
+        }
+        iface_flux<NV, ORD>(c.pf, 2, pll, pl, pr, prr, fzc);
+      }
+      __syncthreads();
+      // conservative update of cell (i, j, k) in FP64
+      double unew[MAXV];
+      {
+        const double *Ub = a.U + (size_t)b * NV * NINT + lin(i, j, k);
+        double ub[MAXV];
+#pragma unroll
+        for (int v = 0; v < NV; v++) ub[v] = Ub[v * NINT];
+        if (c.nwind > 0) {
+          double w[MAXV];
+          if (wind_state(c, b, i, j, k, w))
+#pragma unroll
+            for (int v = 0; v < NV; v++) ub[v] = w[v];
+        }
+#pragma unroll
+        for (int v = 0; v < NV; v++)
+          unew[v] = ub[v] + dtdx * (double)(sfx[v][j - 1][i - 1] - sfx[v][j - 1][i]) +
+                    dtdy * (double)(sfy[v][j - 1][i - 1] - sfy[v][j][i - 1]) + dtdz * (double)(fzp[v] - fzc[v]);
+        if (NV == 8 && c.eight_wave) {
+          float divB = (sp[5][j - 1 + H][i + H] - sp[5][j - 1 + H][i - 2 + H]) * idx2 +
+                       (sp[6][j + H][i - 1 + H] - sp[6][j - 2 + H][i - 1 + H]) * idy2 +
+                       (win[2][7] - win[0][7]) * idz2;
+          const float *pp = win[1];
+          float s1 = -divB * pp[5], s2 = -divB * pp[6], s3 = -divB * pp[7];
+          float s4 = -divB * (pp[1] * pp[5] + pp[2] * pp[6] + pp[3] * pp[7]);
+          float s5 = -divB * pp[1], s6 = -divB * pp[2], s7 = -divB * pp[3];
+          unew[1] += dtp * (double)s1;
+          unew[2] += dtp * (double)s2;
+          unew[3] += dtp * (double)s3;
+          unew[4] += dtp * (double)s4;
+          unew[5] += dtp * (double)s5;
+          unew[6] += dtp * (double)s6;
+          unew[7] += dtp * (double)s7;
+        }
+      }
+#pragma unroll
+      for (int v = 0; v < NV; v++) fzp[v] = fzc[v];
+
+      if (!S2) {
+        float *UPo = a.UPout + (size_t)b * NV * NINT + lin(i, j, k);
+#pragma unroll
+        for (int v = 0; v < NV; v++) UPo[v * NINT] = (float)unew[v];
+        continue;
+      }
+
+      // ---- stage 2 epilogue: viscosity (delayed one plane), prims, cooling, dt
+      int ke = VISC ? k - 1 : k;      // plane finished in this iteration
+      double ufin[MAXV];
+      bool have = true;
+      if (VISC) {
+#pragma unroll
+        for (int v = 0; v < NV; v++) sun[k & 1][v][j][i] = (float)unew[v];
+        __syncthreads();
+        if (k == 1) {
+          have = false;
+        } else {
+          const int pb = (k - 1) & 1;
+#pragma unroll
+          for (int v = 0; v < NV; v++) {
+            double below = (k - 2 == 0) ? ughost_lo[v] : unew_prev2[v];
+            float lap = sun[pb][v][j][i + 1] + sun[pb][v][j][i - 1] + sun[pb][v][j + 1][i] + sun[pb][v][j - 1][i];
+            double l6 = (double)lap + (unew[v] + below) - 6.0 * unew_prev[v];
+            ufin[v] = unew_prev[v] + c.visc_eta * l6;
+          }
+        }
+#pragma unroll
+        for (int v = 0; v < NV; v++) { unew_prev2[v] = unew_prev[v]; unew_prev[v] = unew[v]; }
+      } else {
+#pragma unroll
+        for (int v = 0; v < NV; v++) ufin[v] = unew[v];
+      }
+      if (have) {
+        double pd[MAXV];
+        flow2prim(c.pd, ufin, pd);
+        if (c.cooling) {
+          double e;
+          if (cool_cell(c, dt, pd, &e)) ufin[4] = e;
+        }
+        double *Uo = a.Uout + (size_t)b * NV * NINT + lin(i, j, ke);
+#pragma unroll
+        for (int v = 0; v < NV; v++) Uo[v * NINT] = ufin[v];
+        dtmin = fminf(dtmin, dt_cell<NV>(c, pd, (float)dxl, (float)dyl, (float)dzl));
+      }
+    }
+    if (S2 && VISC) {
+      // plane 16: its upper neighbour is the half-step ghost at plane 17
+      __syncthreads();
+      const int pb = NC & 1;
+      double ufin[MAXV];
+#pragma unroll
+      for (int v = 0; v < NV; v++) {
+        float lap = sun[pb][v][j][i + 1] + sun[pb][v][j][i - 1] + sun[pb][v][j + 1][i] + sun[pb][v][j - 1][i];
+        double l6 = (double)lap + (ughost_hi[v] + unew_prev2[v]) - 6.0 * unew_prev[v];
+        ufin[v] = unew_prev[v] + c.visc_eta * l6;
+      }
+      double pd[MAXV];
+      flow2prim(c.pd, ufin, pd);
+      if (c.cooling) {
+        double e;
+        if (cool_cell(c, dt, pd, &e)) ufin[4] = e;
+      }
+      double *Uo = a.Uout + (size_t)b * NV * NINT + lin(i, j, NC);
+#pragma unroll
+      for (int v = 0; v < NV; v++) Uo[v * NINT] = ufin[v];
+      dtmin = fminf(dtmin, dt_cell<NV>(c, pd, (float)dxl, (float)dyl, (float)dzl));
+    }
+    __syncthreads();
+  }
+  if (S2) {
+    for (int off = 32; off > 0; off >>= 1) dtmin = fminf(dtmin, __shfl_xor(dtmin, off));
+    if ((tid & 63) == 0) sred[tid >> 6] = dtmin;
+    __syncthreads();
+    if (tid == 0) {
+      float m = fminf(fminf(sred[0], sred[1]), fminf(sred[2], sred[3]));
+      double md = (double)m;
+      atomicMin(&a.step->dtmin_bits, (unsigned long long)__double_as_longlong(md));
+    }
+  }
+}
+
+// initial dt candidate from U (after an upload)
+template <int NV>
+__global__ void __launch_bounds__(256) k_dt_init(FusedArgs a) {
+  const DevCfg &c = a.c;
+  int b = blockIdx.x;
+  int l = c.info[b].level - 1;
+  float m = 3.0e38f;
+  for (int q = threadIdx.x; q < NINT; q += 256) {
+    double u[MAXV], pd[MAXV];
+    for (int v = 0; v < NV; v++) u[v] = a.U[((size_t)b * NV + v) * NINT + q];
+    flow2prim(c.pd, u, pd);
+    m = fminf(m, dt_cell<NV>(c, pd, (float)c.dx[0][l], (float)c.dx[1][l], (float)c.dx[2][l]));
+  }
+  for (int off = 32; off > 0; off >>= 1) m = fminf(m, __shfl_xor(m, off));
+  if ((threadIdx.x & 63) == 0) atomicMin(&a.step->dtmin_bits, (unsigned long long)__double_as_longlong((double)m));
+}
+
+// dt of the step: CFL, start-up ramp, output clamp
+// (source/hydro_solver.f90:183-207); resets the running minimum
+__global__ void k_dt_finalize(StepState *s, double cfl, int cooling) {
+  double m = __longlong_as_double((long long)s->dtmin_bits);
+  double dtl = cooling ? fmin(m, 1.0e30) * cfl : m * cfl;
+  int it = s->it + 1;
+  s->it = it;
+  if (it <= 10) dtl = ldexp(1.0, -(11 - it)) * dtl;
+  double tout = (double)s->nextout * s->dtout / s->t_sc;
+  s->dump = 0;
+  if (s->time + dtl >= tout) {
+    dtl = tout - s->time;
+    s->dump = 1;
+  }
+  s->dt = dtl;
+  s->dtmin_bits = ~0ull;
+}
+
+__global__ void k_step_post(StepState *s) {
+  s->time = s->time + s->dt;
+  if (s->dump) s->nextout += 1;
+}
+
+// conversions between the ghosted FP64 layout and the compact one
+__global__ void k_compact(int nv, int nblk, const double *src, double *dst) {
+  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)nblk * nv * NINT) return;
+  int q = (int)(t % NINT);
+  size_t bv = t / NINT;
+  dst[t] = src[bv * NCELL + cid(q % NC + 1, (q / NC) % NC + 1, q / (NC * NC) + 1)];
+}
+template <int NV>
+__global__ void k_expand(const DevCfg c, int nblk, const double *src, double *dstU, double *dstP) {
+  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)nblk * NINT) return;
+  int q = (int)(t % NINT);
+  size_t b = t / NINT;
+  int cq = cid(q % NC + 1, (q / NC) % NC + 1, q / (NC * NC) + 1);
+  double u[MAXV], p[MAXV];
+  for (int v = 0; v < NV; v++) u[v] = src[(b * NV + v) * NINT + q];
+  flow2prim(c.pd, u, p);
+  for (int v = 0; v < NV; v++) {
+    dstU[(b * NV + v) * NCELL + cq] = u[v];
+    dstP[(b * NV + v) * NCELL + cq] = p[v];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+static int hip_fail(hipError_t e, const char *what) {
+  if (e == hipSuccess) return 0;
+  f_err = std::string(what) + ": " + hipGetErrorString(e);
+  return 1;
+}
+#define FCHK(x) do { if (hip_fail((x), #x)) return 1; } while (0)
+
+// Morton key of a block's position at the finest level: neighbours in space
+// end up close in the processing order (shared L2 lines for halo reads)
+static unsigned long long morton(const BlockInfo &B, int maxlev) {
+  unsigned long long key = 0;
+  int shift = maxlev - B.level;
+  unsigned x = (unsigned)(B.coords[0] - 1) << shift, y = (unsigned)(B.coords[1] - 1) << shift,
+           z = (unsigned)(B.coords[2] - 1) << shift;
+  for (int bit = 0; bit < 20; bit++) {
+    key |= (unsigned long long)((x >> bit) & 1) << (3 * bit);
+    key |= (unsigned long long)((y >> bit) & 1) << (3 * bit + 1);
+    key |= (unsigned long long)((z >> bit) & 1) << (3 * bit + 2);
+  }
+  return key;
+}
+
+int fused_alloc(FusedState &f, const DevCfg &c, const std::vector<BlockInfo> &info, hipStream_t st) {
+  fused_free(f);
+  f.nblk = c.nblk;
+  f.nv = c.nv;
+  size_t n = (size_t)c.nblk * c.nv * NINT;
+  FCHK(hipMalloc(&f.U, n * sizeof(double)));
+  FCHK(hipMalloc(&f.UP, n * sizeof(float)));
+  FCHK(hipMalloc(&f.order, c.nblk * sizeof(int)));
+  FCHK(hipMalloc(&f.step, sizeof(StepState)));
+  // processing order: Morton order; with the XCD map, workgroup w runs on
+  // XCD w % 8, so consecutive Morton blocks go to workgroups 8 apart
+  std::vector<std::pair<unsigned long long, int>> keys(c.nblk);
+  for (int b = 0; b < c.nblk; b++) keys[b] = {morton(info[b], c.maxlev), b};
+  std::sort(keys.begin(), keys.end());
+  const int nx = 8;
+  std::vector<int> ord(c.nblk, -1);
+  if (f.xcd_map && c.nblk >= 64) {
+    int full = (c.nblk / nx) * nx;
+    int perx = full / nx;
+    for (int r = 0; r < full; r++) {
+      int xcd = r / perx, within = r % perx;
+      ord[within * nx + xcd] = keys[r].second;
+    }
+    for (int r = full; r < c.nblk; r++) ord[r] = keys[r].second;
+  } else {
+    for (int r = 0; r < c.nblk; r++) ord[r] = keys[r].second;
+  }
+  f.host_order = ord;
+  FCHK(hipMemcpy(f.order, ord.data(), c.nblk * sizeof(int), hipMemcpyHostToDevice));
+  for (int e = 0; e < 4; e++) FCHK(hipEventCreate(&f.ev[e]));
+  (void)st;
+  return 0;
+}
+
+void fused_free(FusedState &f) {
+  if (f.exec) hipGraphExecDestroy(f.exec);
+  if (f.graph) hipGraphDestroy(f.graph);
+  f.exec = nullptr;
+  f.graph = nullptr;
+  hipFree(f.U);
+  hipFree(f.UP);
+  hipFree(f.order);
+  hipFree(f.step);
+  for (int e = 0; e < 4; e++)
+    if (f.ev[e]) hipEventDestroy(f.ev[e]);
+  for (int e = 0; e < 4; e++) f.ev[e] = nullptr;
+  f.U = nullptr;
+  f.UP = nullptr;
+  f.order = nullptr;
+  f.step = nullptr;
+}
+
+static FusedArgs make_args(FusedState &f, const DevCfg &c) {
+  FusedArgs a;
+  a.c = c;
+  a.U = f.U;
+  a.Uout = f.U;
+  a.UPin = f.UP;
+  a.UPout = f.UP;
+  a.order = f.order;
+  a.step = f.step;
+  a.nblk = f.nblk;
+  return a;
+}
+
+static void launch_dt_init(FusedState &f, const DevCfg &c, hipStream_t st) {
+  FusedArgs a = make_args(f, c);
+  if (c.nv == 8) hipLaunchKernelGGL(k_dt_init<8>, dim3(f.nblk), dim3(256), 0, st, a);
+  else hipLaunchKernelGGL(k_dt_init<5>, dim3(f.nblk), dim3(256), 0, st, a);
+}
+
+int fused_upload(FusedState &f, const DevCfg &c, const double *dU, hipStream_t st) {
+  size_t n = (size_t)f.nblk * f.nv * NINT;
+  hipLaunchKernelGGL(k_compact, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, f.nv, f.nblk, dU, f.U);
+  StepState s{};
+  s.dtmin_bits = ~0ull;
+  FCHK(hipMemcpyAsync(f.step, &s, sizeof(s), hipMemcpyHostToDevice, st));
+  launch_dt_init(f, c, st);
+  FCHK(hipGetLastError());
+  FCHK(hipStreamSynchronize(st));
+  return 0;
+}
+
+int fused_download(FusedState &f, const DevCfg &c, double *dU, double *dP, hipStream_t st) {
+  size_t n = (size_t)f.nblk * NINT;
+  if (f.nv == 8)
+    hipLaunchKernelGGL(k_expand<8>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, c, f.nblk, f.U, dU, dP);
+  else
+    hipLaunchKernelGGL(k_expand<5>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, c, f.nblk, f.U, dU, dP);
+  FCHK(hipGetLastError());
+  FCHK(hipStreamSynchronize(st));
+  return 0;
+}
+
+int fused_timestep(FusedState &f, const DevCfg &c, hipStream_t st, double *dt_raw) {
+  StepState s;
+  FCHK(hipMemcpyAsync(&s, f.step, sizeof(s), hipMemcpyDeviceToHost, st));
+  FCHK(hipStreamSynchronize(st));
+  double m;
+  std::memcpy(&m, &s.dtmin_bits, sizeof(m));
+  *dt_raw = c.cooling ? std::fmin(m, 1.0e30) * c.cfl : m * c.cfl;
+  return 0;
+}
+
+static void launch_stage(FusedState &f, const DevCfg &c, hipStream_t st, int stage) {
+  FusedArgs a = make_args(f, c);
+  int grid = f.grid > 0 ? std::min(f.grid, f.nblk) : f.nblk;
+  bool visc = c.visc_eta != 0.0;
+#define WX_LAUNCH(NVv, S2v, Vv) hipLaunchKernelGGL((k_stage<NVv, S2v, Vv>), dim3(grid), dim3(256), 0, st, a)
+  if (c.nv == 8) {
+    if (stage == 1) WX_LAUNCH(8, false, false);
+    else if (visc) WX_LAUNCH(8, true, true);
+    else WX_LAUNCH(8, true, false);
+  } else {
+    if (stage == 1) WX_LAUNCH(5, false, false);
+    else if (visc) WX_LAUNCH(5, true, true);
+    else WX_LAUNCH(5, true, false);
+  }
+#undef WX_LAUNCH
+}
+
+static void enqueue_step(FusedState &f, const DevCfg &c, hipStream_t st) {
+  hipLaunchKernelGGL(k_dt_finalize, dim3(1), dim3(1), 0, st, f.step, c.cfl, c.cooling);
+  launch_stage(f, c, st, 1);
+  launch_stage(f, c, st, 2);
+  hipLaunchKernelGGL(k_step_post, dim3(1), dim3(1), 0, st, f.step);
+}
+
+int fused_run(FusedState &f, const DevCfg &c, hipStream_t st, int nsteps, int *it, double *dt, double *time,
+              int *nextout, double dtout, double t_sc) {
+  if (nsteps == 0) return 0;
+  // host scalars -> device (keeps the running dt minimum)
+  StepState s;
+  FCHK(hipMemcpyAsync(&s, f.step, sizeof(s), hipMemcpyDeviceToHost, st));
+  FCHK(hipStreamSynchronize(st));
+  s.it = *it;
+  s.time = *time;
+  s.nextout = *nextout;
+  s.dtout = dtout;
+  s.t_sc = t_sc;
+  FCHK(hipMemcpyAsync(f.step, &s, sizeof(s), hipMemcpyHostToDevice, st));
+  if (!f.exec) {
+    FCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    enqueue_step(f, c, st);
+    hipError_t e = hipStreamEndCapture(st, &f.graph);
+    FCHK(e);
+    FCHK(hipGraphInstantiate(&f.exec, f.graph, nullptr, nullptr, 0));
+  }
+  FCHK(hipEventRecord(f.ev[0], st));
+  for (int n = 0; n < nsteps; n++) FCHK(hipGraphLaunch(f.exec, st));
+  FCHK(hipEventRecord(f.ev[1], st));
+  FCHK(hipMemcpyAsync(&s, f.step, sizeof(s), hipMemcpyDeviceToHost, st));
+  FCHK(hipStreamSynchronize(st));
+  float ms = 0;
+  hipEventElapsedTime(&ms, f.ev[0], f.ev[1]);
+  f.t_tot = nsteps > 0 ? ms / nsteps : 0.0;
+  *it = s.it;
+  *dt = s.dt;
+  *time = s.time;
+  *nextout = s.nextout;
+  return 0;
+}
+
+// Times the two stage kernels separately (HIP events on the library stream)
+// by replaying one step outside the graph; used by the benchmark.
+int fused_time_stages(FusedState &f, const DevCfg &c, hipStream_t st, int reps) {
+  float a1 = 0, a2 = 0;
+  for (int r = 0; r < reps; r++) {
+    hipLaunchKernelGGL(k_dt_finalize, dim3(1), dim3(1), 0, st, f.step, c.cfl, c.cooling);
+    FCHK(hipEventRecord(f.ev[0], st));
+    launch_stage(f, c, st, 1);
+    FCHK(hipEventRecord(f.ev[1], st));
+    launch_stage(f, c, st, 2);
+    FCHK(hipEventRecord(f.ev[2], st));
+    hipLaunchKernelGGL(k_step_post, dim3(1), dim3(1), 0, st, f.step);
+    FCHK(hipEventSynchronize(f.ev[2]));
+    float m1, m2;
+    hipEventElapsedTime(&m1, f.ev[0], f.ev[1]);
+    hipEventElapsedTime(&m2, f.ev[1], f.ev[2]);
+    a1 += m1;
+    a2 += m2;
+  }
+  f.t_s1 = a1 / reps;
+  f.t_s2 = a2 / reps;
+  f.timing_runs = reps;
+  return 0;
+}
+
+int fused_set_tuning(FusedState &f, int kind, int value) {
+  if (kind == 0) { f.grid = value; if (f.exec) { hipGraphExecDestroy(f.exec); hipGraphDestroy(f.graph); f.exec = nullptr; f.graph = nullptr; } return 0; }
+  return 1;
+}
+
+void fused_last_timing(const FusedState &f, double *s1, double *s2, double *tot) {
+  if (s1) *s1 = f.t_s1;
+  if (s2) *s2 = f.t_s2;
+  if (tot) *tot = f.t_tot;
+}
+
+}  // namespace wx

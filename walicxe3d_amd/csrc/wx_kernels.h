@@ -1,0 +1,21 @@
+// wx_kernels.h - host-side launchers of the device kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "wx_device.h"
+
+namespace wx {
+
+// plain FP64 form (wx_plain.hip)
+void plain_boundary(const DevCfg &c, double *A, int depth, hipStream_t st);
+void plain_prims(const DevCfg &c, const double *A, double *P, int cells, unsigned long long *bad, hipStream_t st);
+void plain_fluxes(const DevCfg &c, const double *P, double *F, int order, hipStream_t st);
+void plain_upwind(const DevCfg &c, const double *U, const double *P, const double *F, double *UP, double dtp,
+                  hipStream_t st);
+void plain_viscous(const DevCfg &c, const double *UP, double *U, hipStream_t st);
+void plain_timestep(const DevCfg &c, const double *P, unsigned long long *dtbits, hipStream_t st);
+void plain_cooling(const DevCfg &c, double *U, double *P, double dt, hipStream_t st);
+size_t plain_flux_elems(int nblk, int nv);
+
+}  // namespace wx

@@ -1,0 +1,368 @@
+// wx_physics.h - cell and interface physics of Walicxe3D's Godunov update,
+// written once as templates over the arithmetic type R:
+//   R = double : the plain form; every expression keeps the reference's
+//                evaluation order (Fortran left to right) and the library is
+//                built with -ffp-contract=off, so results match the FP64
+//                reference bit for bit (IEEE division and square root).
+//   R = float  : the bulk arithmetic of the fused form.
+// Reference routines are cited per function.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "wx_common.h"
+
+namespace wx {
+
+template <typename R> struct Phys {
+  int nv;        // 5 or 8
+  int mhd;
+  int solver;
+  int limiter;
+  R gamma, cv;
+};
+
+template <typename R> __host__ __device__ inline R rabs(R a) { return a < R(0) ? -a : a; }
+__host__ __device__ inline double rsqrt_(double a) { return sqrt(a); }
+__host__ __device__ inline float rsqrt_(float a) { return sqrtf(a); }
+__host__ __device__ inline double rfmin(double a, double b) { return fmin(a, b); }
+__host__ __device__ inline float rfmin(float a, float b) { return fminf(a, b); }
+__host__ __device__ inline double rfmax(double a, double b) { return fmax(a, b); }
+__host__ __device__ inline float rfmax(float a, float b) { return fmaxf(a, b); }
+__host__ __device__ inline double rcopysign(double a, double b) { return copysign(a, b); }
+__host__ __device__ inline float rcopysign(float a, float b) { return copysignf(a, b); }
+
+// flow2prim (source/hydro_core.f90:323-378); returns istat (0, 1 p floor, 2 rho floor)
+template <typename R>
+__host__ __device__ inline int flow2prim(const Phys<R> &c, const R *u, R *p) {
+  int istat = 0;
+  p[0] = u[0];
+  p[1] = u[1] / u[0];
+  p[2] = u[2] / u[0];
+  p[3] = u[3] / u[0];
+  p[4] = (u[4] - R(0.5) * (u[1] * u[1] + u[2] * u[2] + u[3] * u[3]) / u[0]) / c.cv;
+  if (c.nv == 8 && c.mhd) p[4] = p[4] - R(0.5) * (u[5] * u[5] + u[6] * u[6] + u[7] * u[7]) / c.cv;
+  if (p[4] < R(1.0e-30)) { p[4] = R(1.0e-30); istat = 1; }
+  if (p[0] < R(1.0e-40)) { p[0] = R(1.0e-40); istat = 2; }
+  if (c.nv == 8) { p[5] = u[5]; p[6] = u[6]; p[7] = u[7]; }
+  return istat;
+}
+
+// prim2flow (source/hydro_core.f90:278-313)
+template <typename R>
+__host__ __device__ inline void prim2flow(const Phys<R> &c, const R *p, R *u) {
+  u[0] = p[0];
+  u[1] = p[0] * p[1];
+  u[2] = p[0] * p[2];
+  u[3] = p[0] * p[3];
+  R v2 = p[1] * p[1] + p[2] * p[2] + p[3] * p[3];
+  u[4] = R(0.5) * p[0] * v2 + c.cv * p[4];
+  if (c.nv == 8) {
+    if (c.mhd) u[4] = u[4] + R(0.5) * (p[5] * p[5] + p[6] * p[6] + p[7] * p[7]);
+    u[5] = p[5];
+    u[6] = p[6];
+    u[7] = p[7];
+  }
+}
+
+// x-direction physical flux of a primitive state (prim2fluxes with
+// dimens = DIM_X, source/hydro_core.f90:387-442).  Callers rotate states
+// and fluxes for y and z (swapxy/swapxz, :448-496).
+template <typename R>
+__host__ __device__ inline void xflux(const Phys<R> &c, const R *p, R *f) {
+  f[0] = p[0] * p[1];
+  f[1] = p[0] * (p[1] * p[1]) + p[4];
+  f[2] = p[0] * p[1] * p[2];
+  f[3] = p[0] * p[1] * p[3];
+  f[4] = p[1] * (R(0.5) * p[0] * (p[1] * p[1] + p[2] * p[2] + p[3] * p[3]) + (c.cv + R(1.0)) * p[4]);
+  if (c.nv == 8) {
+    f[5] = R(0);
+    f[6] = p[1] * p[6] - p[2] * p[5];
+    f[7] = p[1] * p[7] - p[3] * p[5];
+    if (c.mhd) {
+      f[1] = f[1] + R(0.5) * (p[6] * p[6] + p[7] * p[7] - p[5] * p[5]);
+      f[2] = f[2] - p[5] * p[6];
+      f[3] = f[3] - p[5] * p[7];
+      f[4] = f[4] + p[1] * (p[5] * p[5] + p[6] * p[6] + p[7] * p[7]) -
+             p[5] * (p[1] * p[5] + p[2] * p[6] + p[3] * p[7]);
+    }
+  }
+}
+
+// swap of the x and d components of velocity and field (d = 1 -> y, 2 -> z)
+template <typename R> __host__ __device__ inline void swapdim(int nv, int d, R *v) {
+  if (d == 0) return;
+  R t = v[1]; v[1] = v[1 + d]; v[1 + d] = t;
+  if (nv == 8) { t = v[5]; v[5] = v[5 + d]; v[5 + d] = t; }
+}
+
+// sound (source/hydro_core.f90:503-515)
+template <typename R> __host__ __device__ inline R sound(const Phys<R> &c, const R *p) {
+  return rsqrt_(c.gamma * p[4] / p[0]);
+}
+
+// cfast (source/hydro_core.f90:530-543): fast speed along one axis with
+// field component bd
+template <typename R>
+__host__ __device__ inline R cfast1(const Phys<R> &c, R p, R d, R bx, R by, R bz, R bd) {
+  R b2 = bx * bx + by * by + bz * bz;
+  R a = c.gamma * p + b2;
+  return rsqrt_(R(0.5) * (a + rsqrt_(a * a - R(4.0) * c.gamma * p * bd * bd)) / d);
+}
+
+// cfastX (source/hydro_core.f90:553-566)
+template <typename R> __host__ __device__ inline R cfastX(const Phys<R> &c, const R *p) {
+  R b2 = p[5] * p[5] + p[6] * p[6] + p[7] * p[7];
+  R cs2va2 = (c.gamma * p[4] + b2) / p[0];
+  return rsqrt_(R(0.5) *
+                (cs2va2 + rsqrt_(cs2va2 * cs2va2 - R(4.0) * c.gamma * p[4] * (p[5] * p[5]) / p[0] / p[0])));
+}
+
+// average (source/hydro_core.f90:640-697)
+template <typename R> __host__ __device__ inline R average(R a, R b, int opt) {
+  R s, cc, d, r;
+  switch (opt) {
+    case -1: return R(0);
+    case 0: return R(0.5) * (a + b);
+    case 2:
+      if (a * b <= R(0)) return R(0);
+      return a * b * (a + b) / (a * a + b * b);
+    case 1:
+      s = rcopysign(R(1), a);
+      return s * rfmax(R(0), rfmin(rabs(a), s * b));
+    case 3: {
+      R eps = R(1.0e-7);
+      return (a * (b * b + eps) + b * (a * a + eps)) / (a * a + b * b * eps);
+    }
+    case 4:
+      s = rcopysign(R(1), a);
+      cc = R(0.25) * a + R(0.75) * b;
+      d = R(0.75) * a + R(0.25) * b;
+      r = rfmin(rfmin(rfmin(R(2.0) * rabs(a), R(2.0) * s * b), s * cc), s * d);
+      return s * rfmax(R(0), r);
+    case 5:
+      s = rcopysign(R(1), a);
+      cc = R(0.5) * (a + b);
+      r = rfmin(rfmin(R(2.0) * rabs(a), R(2.0) * s * b), s * cc);
+      return s * rfmax(R(0), r);
+    case 6:
+      s = rcopysign(R(1), b);
+      cc = rfmin(R(2.0) * rabs(b), s * a);
+      d = rfmin(rabs(b), R(2.0) * s * a);
+      return s * rfmax(rfmax(R(0), cc), d);
+    default: return R(0);
+  }
+}
+
+// limiter (source/hydro_core.f90:613-638) for one variable
+template <typename R>
+__host__ __device__ inline void limit1(R pll, R &pl, R &pr, R prr, int lim) {
+  R dl = pl - pll, dm = pr - pl, dr = prr - pr;
+  R al = average(dl, dm, lim), ar = average(dm, dr, lim);
+  pl = pl + R(0.5) * al;
+  pr = pr - R(0.5) * ar;
+}
+
+// HLL / HLLE flux (source/hll.f90:175-215, source/hlle.f90:166-206);
+// strict selects the HLLE comparisons (sl > 0, sr < 0)
+template <typename R>
+__host__ __device__ inline void hll_flux(const Phys<R> &c, const R *pL, const R *pR, R sl, R sr, bool strict,
+                                         R *ff) {
+  R fL[MAXV], fR[MAXV], uL[MAXV], uR[MAXV];
+  bool left = strict ? (sl > R(0)) : (sl >= R(0));
+  bool right = strict ? (sr < R(0)) : (sr <= R(0));
+  if (left) { xflux(c, pL, ff); return; }
+  if (right) { xflux(c, pR, ff); return; }
+  xflux(c, pL, fL);
+  xflux(c, pR, fR);
+  prim2flow(c, pL, uL);
+  prim2flow(c, pR, uR);
+  for (int q = 0; q < c.nv; q++) ff[q] = (sr * fL[q] - sl * fR[q] + sl * sr * (uR[q] - uL[q])) / (sr - sl);
+}
+
+// primfhll + wavespeedHLL (source/hll.f90:175-260)
+template <typename R>
+__host__ __device__ inline void primfhll(const Phys<R> &c, const R *pL, const R *pR, R *ff) {
+  R cl = sound(c, pL), cr = sound(c, pR);
+  R sl = rfmin(pL[1] - cl, pR[1] - cr);
+  R sr = rfmax(pL[1] + cl, pR[1] + cr);
+  hll_flux(c, pL, pR, sl, sr, false, ff);
+}
+
+// primfhlle + wavespeedHLLE (source/hlle.f90:166-241)
+template <typename R>
+__host__ __device__ inline void primfhlle(const Phys<R> &c, const R *pL, const R *pR, R *ff) {
+  R cfL = cfastX(c, pL), cfR = cfastX(c, pR);
+  R sl = rfmin(pL[1] - cfL, pR[1] - cfR);
+  R sr = rfmax(pL[1] + cfL, pR[1] + cfR);
+  hll_flux(c, pL, pR, sl, sr, true, ff);
+}
+
+// primfhllc + wavespeedHLLC (source/hllc.f90:209-357), Toro 10.33-10.38
+template <typename R>
+__host__ __device__ inline void primfhllc(const Phys<R> &c, const R *pL, const R *pR, R *ff) {
+  R dl1 = pL[0], dr1 = pR[0], ul1 = pL[1], ur1 = pR[1], pl1 = pL[4], pr1 = pR[4];
+  R cl = sound(c, pL), cr = sound(c, pR);
+  R sl = rfmin(ul1 - cl, ur1 - cr);
+  R sr = rfmax(ul1 + cl, ur1 + cr);
+  R sst;
+  if (c.solver == SOLVER_HLLC)
+    sst = (pr1 - pl1 + dl1 * ul1 * (sl - ul1) - dr1 * ur1 * (sr - ur1)) / (dl1 * (sl - ul1) - dr1 * (sr - ur1));
+  else
+    sst = R(0);
+  if (sl > R(0)) { xflux(c, pL, ff); return; }
+  if (sr < R(0)) { xflux(c, pR, ff); return; }
+  const R *pK;
+  R sK;
+  if (sst >= R(0)) { pK = pL; sK = sl; }
+  else if (sst <= R(0)) { pK = pR; sK = sr; }
+  else { for (int q = 0; q < c.nv; q++) ff[q] = R(NAN); return; }
+  R fK[MAXV], uK[MAXV], ust[MAXV];
+  xflux(c, pK, fK);
+  prim2flow(c, pK, uK);
+  R rhost = pK[0] * (sK - pK[1]) / (sK - sst);
+  R EK = R(0.5) * pK[0] * (pK[1] * pK[1] + pK[2] * pK[2] + pK[3] * pK[3]) + c.cv * pK[4];
+  ust[0] = rhost;
+  ust[1] = rhost * sst;
+  ust[2] = rhost * pK[2];
+  ust[3] = rhost * pK[3];
+  ust[4] = rhost * (EK / pK[0] + (sst - pK[1]) * (sst + pK[4] / (pK[0] * (sK - pK[1]))));
+  for (int q = 5; q < c.nv; q++) ust[q] = R(0);
+  for (int q = 0; q < c.nv; q++) ff[q] = fK[q] + sK * (ust[q] - uK[q]);
+}
+
+// star state of one side for HLLD (source/hlld.f90:241-257, 289-305, 337-365)
+template <typename R>
+__host__ __device__ inline void hlld_star(const R *p, R bx, R smuK, R smsM, R sMmuK, R &vst, R &wst, R &byst,
+                                          R &bzst) {
+  R den = p[0] * smuK * smsM - bx * bx;
+  if (den == R(0)) {
+    vst = p[2]; wst = p[3]; byst = R(0); bzst = R(0);
+  } else {
+    vst = p[2] - bx * p[6] * sMmuK / den;
+    wst = p[3] - bx * p[7] * sMmuK / den;
+    byst = p[6] * (p[0] * (smuK * smuK) - bx * bx) / den;
+    bzst = p[7] * (p[0] * (smuK * smuK) - bx * bx) / den;
+  }
+}
+
+// primfhlld (source/hlld.f90:166-444), Miyoshi & Kusano (2005)
+template <typename R>
+__host__ __device__ inline void primfhlld(const Phys<R> &c, const R *pl, const R *pr, R *ff) {
+  const R cv = c.cv;
+  R csl = cfastX(c, pl), csr = cfastX(c, pr);
+  R sr = rfmax(pl[1] + csl, pr[1] + csr);
+  R sl = rfmin(pl[1] - csl, pr[1] - csr);
+  if (sl > R(0)) { xflux(c, pl, ff); return; }
+  if (sr < R(0)) { xflux(c, pr, ff); return; }
+  R bx = R(0.5) * (pl[5] + pr[5]);
+  R signBx = rcopysign(R(1), bx);
+  R pTL = pl[4] + R(0.5) * (bx * bx + pl[6] * pl[6] + pl[7] * pl[7]);
+  R pTR = pr[4] + R(0.5) * (bx * bx + pr[6] * pr[6] + pr[7] * pr[7]);
+  R slmul = sl - pl[1];
+  R srmur = sr - pr[1];
+  R rholul = pl[0] * pl[1];
+  R rhorur = pr[0] * pr[1];
+  R sM = (srmur * rhorur - slmul * rholul - pTR + pTL) / (srmur * pr[0] - slmul * pl[0]);
+  R srmsM = sr - sM;
+  R slmsM = sl - sM;
+  R rhostl = pl[0] * slmul / slmsM;
+  R rhostr = pr[0] * srmur / srmsM;
+  R sstl = sM - rabs(bx) / rsqrt_(rhostl);
+  R sstr = sM + rabs(bx) / rsqrt_(rhostr);
+  R pst = (srmur * pr[0] * pTL - slmul * pl[0] * pTR + pl[0] * pr[0] * srmur * slmul * (pr[1] - pl[1])) /
+          (srmur * pr[0] - slmul * pl[0]);
+  R vstl, wstl, bystl, bzstl, vstr, wstr, bystr, bzstr;
+  if (sstl >= R(0)) {  // UL*
+    R el = R(0.5) * pl[0] * (pl[1] * pl[1] + pl[2] * pl[2] + pl[3] * pl[3]) + cv * pl[4] +
+           R(0.5) * (bx * bx + pl[6] * pl[6] + pl[7] * pl[7]);
+    hlld_star(pl, bx, slmul, slmsM, sM - pl[1], vstl, wstl, bystl, bzstl);
+    R vdotbl = pl[1] * bx + pl[2] * pl[6] + pl[3] * pl[7];
+    R vstdotbstl = sM * bx + vstl * bystl + wstl * bzstl;
+    R estl = (slmul * el - pTL * pl[1] + pst * sM + bx * (vdotbl - vstdotbstl)) / slmsM;
+    ff[0] = rhostl * sM;
+    ff[1] = rhostl * (sM * sM) + pst - bx * bx;
+    ff[2] = rhostl * sM * vstl - bx * bystl;
+    ff[3] = rhostl * sM * wstl - bx * bzstl;
+    ff[4] = sM * (estl + pst) - bx * (vstdotbstl);
+    ff[5] = R(0);
+    ff[6] = bystl * sM - bx * vstl;
+    ff[7] = bzstl * sM - bx * wstl;
+    return;
+  }
+  if (sstr <= R(0)) {  // UR*
+    R er = R(0.5) * pr[0] * (pr[1] * pr[1] + pr[2] * pr[2] + pr[3] * pr[3]) + cv * pr[4] +
+           R(0.5) * (bx * bx + pr[6] * pr[6] + pr[7] * pr[7]);
+    hlld_star(pr, bx, srmur, srmsM, sM - pr[1], vstr, wstr, bystr, bzstr);
+    R vdotbr = pr[1] * bx + pr[2] * pr[6] + pr[3] * pr[7];
+    R vstdotbstr = sM * bx + vstr * bystr + wstr * bzstr;
+    R estr = (srmur * er - pTR * pr[1] + pst * sM + bx * (vdotbr - vstdotbstr)) / srmsM;
+    ff[0] = rhostr * sM;
+    ff[1] = rhostr * (sM * sM) + pst - bx * bx;
+    ff[2] = rhostr * sM * vstr - bx * bystr;
+    ff[3] = rhostr * sM * wstr - bx * bzstr;
+    ff[4] = sM * (estr + pst) - bx * (vstdotbstr);
+    ff[5] = R(0);
+    ff[6] = bystr * sM - bx * vstr;
+    ff[7] = bzstr * sM - bx * wstr;
+    return;
+  }
+  hlld_star(pl, bx, slmul, slmsM, sM - pl[1], vstl, wstl, bystl, bzstl);
+  hlld_star(pr, bx, srmur, srmsM, sM - pr[1], vstr, wstr, bystr, bzstr);
+  R sql = rsqrt_(rhostl), sqr = rsqrt_(rhostr);
+  R dd = sql + sqr;
+  R vstst = (sql * vstl + sqr * vstr + (bystr - bystl) * signBx) / dd;
+  R wstst = (sql * wstl + sqr * wstr + (bzstr - bzstl) * signBx) / dd;
+  R sqlr = rsqrt_(rhostl * rhostr);
+  R bystst = (sql * bystr + sqr * bystl + sqlr * (vstr - vstl) * signBx) / dd;
+  R bzstst = (sql * bzstr + sqr * bzstl + sqlr * (wstr - wstl) * signBx) / dd;
+  R vststdotbstst = sM * bx + vstst * bystst + wstst * bzstst;
+  if (sM >= R(0)) {  // UL**
+    R el = R(0.5) * pl[0] * (pl[1] * pl[1] + pl[2] * pl[2] + pl[3] * pl[3]) + cv * pl[4] +
+           R(0.5) * (bx * bx + pl[6] * pl[6] + pl[7] * pl[7]);
+    R vdotbl = pl[1] * bx + pl[2] * pl[6] + pl[3] * pl[7];
+    R vstdotbstl = sM * bx + vstl * bystl + wstl * bzstl;
+    R estl = (slmul * el - pTL * pl[1] + pst * sM + bx * (vdotbl - vstdotbstl)) / slmsM;
+    R eststl = estl - sql * (vstdotbstl - vststdotbstst) * signBx;
+    ff[0] = rhostl * sM;
+    ff[1] = rhostl * (sM * sM) + pst - bx * bx;
+    ff[2] = rhostl * sM * vstst - bx * bystst;
+    ff[3] = rhostl * sM * wstst - bx * bzstst;
+    ff[4] = sM * (eststl + pst) - bx * (vststdotbstst);
+    ff[5] = R(0);
+    ff[6] = bystst * sM - bx * vstst;
+    ff[7] = bzstst * sM - bx * wstst;
+    return;
+  }
+  if (sM <= R(0)) {  // UR**
+    R er = R(0.5) * pr[0] * (pr[1] * pr[1] + pr[2] * pr[2] + pr[3] * pr[3]) + cv * pr[4] +
+           R(0.5) * (bx * bx + pr[6] * pr[6] + pr[7] * pr[7]);
+    R vdotbr = pr[1] * bx + pr[2] * pr[6] + pr[3] * pr[7];
+    R vstdotbstr = sM * bx + vstr * bystr + wstr * bzstr;
+    R estr = (srmur * er - pTR * pr[1] + pst * sM + bx * (vdotbr - vstdotbstr)) / srmsM;
+    R eststr = estr + sqr * (vstdotbstr - vststdotbstst) * signBx;
+    ff[0] = rhostr * sM;
+    ff[1] = rhostr * (sM * sM) + pst - bx * bx;
+    ff[2] = rhostr * sM * vstst - bx * bystst;
+    ff[3] = rhostr * sM * wstst - bx * bzstst;
+    ff[4] = sM * (eststr + pst) - bx * (vststdotbstst);
+    ff[5] = R(0);
+    ff[6] = bystst * sM - bx * vstst;
+    ff[7] = bzstst * sM - bx * wstst;
+    return;
+  }
+  for (int q = 0; q < 8; q++) ff[q] = R(NAN);
+}
+
+// solver dispatch on x-aligned states
+template <typename R>
+__host__ __device__ inline void riemann(const Phys<R> &c, const R *pl, const R *pr, R *ff) {
+  switch (c.solver) {
+    case SOLVER_HLLC: primfhllc(c, pl, pr, ff); break;
+    case SOLVER_HLLD: primfhlld(c, pl, pr, ff); break;
+    case SOLVER_HLLE: primfhlle(c, pl, pr, ff); break;
+    default: primfhll(c, pl, pr, ff); break;
+  }
+}
+
+}  // namespace wx

@@ -1,0 +1,288 @@
+// wx_plain.hip - the plain FP64 form: one kernel per reference routine,
+// arrays with two ghost layers per block ([blk][nv][20^3], i fastest),
+// arithmetic in the reference's order (built with -ffp-contract=off).
+#include <hip/hip_runtime.h>
+
+#include "wx_device.h"
+#include "wx_kernels.h"
+
+namespace wx {
+
+constexpr int NIF = 17 * NC * NC;   // interfaces per direction per block used by upwindStep
+
+// ---------------------------------------------------------------------------
+// boundary (source/boundary.f90:46-64): face ghosts of depth `depth`.
+// grid (6*depth, nblk), block 256 = transverse 16x16.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_boundary(DevCfg c, double *A, int depth) {
+  int b = blockIdx.y;
+  int f = blockIdx.x / depth;
+  int g = blockIdx.x % depth + 1;
+  int a1 = threadIdx.x % NC + 1, a2 = threadIdx.x / NC + 1;
+  GhostSrc s;
+  if (!ghost_source(c.info, c.bc, b, f, g, a1, a2, s)) return;
+  int n, t1, t2;
+  face_axes(f, n, t1, t2);
+  int d[3];
+  d[n] = (f & 1) ? NC + g : 1 - g;
+  d[t1] = a1;
+  d[t2] = a2;
+  int q = cid(d[0], d[1], d[2]);
+  int nv = c.nv;
+  double val[MAXV];
+  for (int v = 0; v < nv; v++) val[v] = ghost_value<double>(A, nv, s, v);
+  for (int v = 0; v < nv; v++) A[((size_t)b * nv + v) * NCELL + q] = val[v];
+}
+
+// Wind sources over interior and face-ghost cells (userBoundary).
+// grid (28, nblk): 16 blocks of 256 interior cells, 12 of face ghosts.
+__global__ void __launch_bounds__(256) k_winds(DevCfg c, double *A) {
+  int b = blockIdx.y;
+  int i, j, k;
+  if (blockIdx.x < 16) {
+    int t = blockIdx.x * 256 + threadIdx.x;
+    i = t % NC + 1; j = (t / NC) % NC + 1; k = t / (NC * NC) + 1;
+  } else {
+    int f = (blockIdx.x - 16) / 2, g = (blockIdx.x - 16) % 2 + 1;
+    int n, t1, t2, d[3];
+    face_axes(f, n, t1, t2);
+    d[n] = (f & 1) ? NC + g : 1 - g;
+    d[t1] = threadIdx.x % NC + 1;
+    d[t2] = threadIdx.x / NC + 1;
+    i = d[0]; j = d[1]; k = d[2];
+  }
+  double u[MAXV];
+  if (wind_state(c, b, i, j, k, u)) {
+    int q = cid(i, j, k);
+    for (int v = 0; v < c.nv; v++) A[((size_t)b * c.nv + v) * NCELL + q] = u[v];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// calcPrimsAll (source/hydro_core.f90:67-223): interior (cells ALL/PHYS)
+// and face ghosts with non-zero density (ALL/GHOST).  Edge and corner
+// ghosts are never read by the update and are skipped.
+// grid (28, nblk)
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_prims(DevCfg c, const double *A, double *P, int cells,
+                                               unsigned long long *bad) {
+  int b = blockIdx.y;
+  int i, j, k;
+  bool ghost = blockIdx.x >= 16;
+  if (!ghost) {
+    if (cells == 2) return;
+    int t = blockIdx.x * 256 + threadIdx.x;
+    i = t % NC + 1; j = (t / NC) % NC + 1; k = t / (NC * NC) + 1;
+  } else {
+    if (cells == 1) return;
+    int f = (blockIdx.x - 16) / 2, g = (blockIdx.x - 16) % 2 + 1;
+    int n, t1, t2, d[3];
+    face_axes(f, n, t1, t2);
+    d[n] = (f & 1) ? NC + g : 1 - g;
+    d[t1] = threadIdx.x % NC + 1;
+    d[t2] = threadIdx.x / NC + 1;
+    i = d[0]; j = d[1]; k = d[2];
+  }
+  int nv = c.nv, q = cid(i, j, k);
+  const double *Ab = A + (size_t)b * nv * NCELL;
+  double u[MAXV], p[MAXV];
+  if (ghost && Ab[q] == 0.0) return;
+  for (int v = 0; v < nv; v++) u[v] = Ab[v * NCELL + q];
+  int st = flow2prim(c.pd, u, p);
+  double *Pb = P + (size_t)b * nv * NCELL;
+  for (int v = 0; v < nv; v++) Pb[v * NCELL + q] = p[v];
+  if (st && bad && !ghost) atomicAdd(bad, 1ull);
+}
+
+// ---------------------------------------------------------------------------
+// HLL*fluxes (hll.f90:52-167, hllc.f90:55-201, hlle.f90:54-158,
+// hlld.f90:54-158): the 3 x 17 x 16 x 16 interfaces upwindStep reads.
+// grid (3*17, nblk), block 256; F layout [blk][dim][nv][17*16*16].
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_fluxes(DevCfg c, const double *P, double *F, int order) {
+  int b = blockIdx.y;
+  int dim = blockIdx.x / 17;
+  int s = blockIdx.x % 17;          // position along dim: interface (s | s+1)
+  int a1 = threadIdx.x % NC + 1, a2 = threadIdx.x / NC + 1;
+  int n, t1, t2;
+  face_axes(2 * dim, n, t1, t2);
+  int base[3];
+  base[n] = s;
+  base[t1] = a1;
+  base[t2] = a2;
+  int nv = c.nv;
+  const double *Pb = P + (size_t)b * nv * NCELL;
+  int st = (dim == 0) ? 1 : (dim == 1) ? NB : NB * NB;
+  int q0 = cid(base[0], base[1], base[2]);
+  double pl[MAXV], pr[MAXV], ff[MAXV];
+  for (int v = 0; v < nv; v++) {
+    pl[v] = Pb[v * NCELL + q0];
+    pr[v] = Pb[v * NCELL + q0 + st];
+  }
+  swapdim(nv, dim, pl);
+  swapdim(nv, dim, pr);
+  if (order == 2) {
+    double pll[MAXV], prr[MAXV];
+    for (int v = 0; v < nv; v++) {
+      pll[v] = Pb[v * NCELL + q0 - st];
+      prr[v] = Pb[v * NCELL + q0 + 2 * st];
+    }
+    swapdim(nv, dim, pll);
+    swapdim(nv, dim, prr);
+    for (int v = 0; v < nv; v++) limit1(pll[v], pl[v], pr[v], prr[v], c.pd.limiter);
+  }
+  riemann(c.pd, pl, pr, ff);
+  swapdim(nv, dim, ff);
+  // interface index: position s fastest along dim, then t1, then t2
+  int idx = ((a2 - 1) * NC + (a1 - 1)) * 17 + s;
+  double *Fb = F + ((size_t)b * 3 + dim) * nv * NIF;
+  for (int v = 0; v < nv; v++) Fb[(size_t)v * NIF + idx] = ff[v];
+}
+
+__device__ inline int fidx(int s, int a1, int a2) { return ((a2 - 1) * NC + (a1 - 1)) * 17 + s; }
+
+// ---------------------------------------------------------------------------
+// upwindStep (source/godunov.f90:199-277) with source_function
+// (source/sources.f90:100-133, Powell terms :38-95).  grid (16, nblk).
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_upwind(DevCfg c, const double *U, const double *P, const double *F,
+                                                double *UP, double dtp) {
+  int b = blockIdx.y;
+  int t = blockIdx.x * 256 + threadIdx.x;
+  int i = t % NC + 1, j = (t / NC) % NC + 1, k = t / (NC * NC) + 1;
+  int nv = c.nv, l = c.info[b].level - 1;
+  double dx = c.dx[0][l], dy = c.dx[1][l], dz = c.dx[2][l];
+  double dtdx = dtp / dx, dtdy = dtp / dy, dtdz = dtp / dz;
+  const double *Fx = F + ((size_t)b * 3 + 0) * nv * NIF;
+  const double *Fy = F + ((size_t)b * 3 + 1) * nv * NIF;
+  const double *Fz = F + ((size_t)b * 3 + 2) * nv * NIF;
+  int q = cid(i, j, k);
+  const double *Ub = U + (size_t)b * nv * NCELL;
+  double *UPb = UP + (size_t)b * nv * NCELL;
+  // x: (s=i, t1=j, t2=k); y: (s=j, t1=i, t2=k); z: (s=k, t1=i, t2=j)
+  int ix0 = fidx(i - 1, j, k), ix1 = fidx(i, j, k);
+  int iy0 = fidx(j - 1, i, k), iy1 = fidx(j, i, k);
+  int iz0 = fidx(k - 1, i, j), iz1 = fidx(k, i, j);
+  double upv[MAXV];
+  for (int v = 0; v < nv; v++) {
+    size_t o = (size_t)v * NIF;
+    upv[v] = Ub[v * NCELL + q] + dtdx * (Fx[o + ix0] - Fx[o + ix1]) + dtdy * (Fy[o + iy0] - Fy[o + iy1]) +
+             dtdz * (Fz[o + iz0] - Fz[o + iz1]);
+  }
+  if (c.eight_wave && nv == 8) {
+    const double *Pb = P + (size_t)b * nv * NCELL;
+    double divB = (Pb[5 * NCELL + q + 1] - Pb[5 * NCELL + q - 1]) / (2.0 * dx) +
+                  (Pb[6 * NCELL + q + NB] - Pb[6 * NCELL + q - NB]) / (2.0 * dy) +
+                  (Pb[7 * NCELL + q + NB * NB] - Pb[7 * NCELL + q - NB * NB]) / (2.0 * dz);
+    double pp[MAXV];
+    for (int v = 0; v < nv; v++) pp[v] = Pb[v * NCELL + q];
+    double s[MAXV] = {0, 0, 0, 0, 0, 0, 0, 0};
+    s[1] = s[1] - divB * pp[5];
+    s[2] = s[2] - divB * pp[6];
+    s[3] = s[3] - divB * pp[7];
+    s[4] = s[4] - divB * (pp[1] * pp[5] + pp[2] * pp[6] + pp[3] * pp[7]);
+    s[5] = s[5] - divB * pp[1];
+    s[6] = s[6] - divB * pp[2];
+    s[7] = s[7] - divB * pp[3];
+    for (int v = 0; v < nv; v++) upv[v] = upv[v] + dtp * s[v];
+  }
+  for (int v = 0; v < nv; v++) UPb[v * NCELL + q] = upv[v];
+}
+
+// ---------------------------------------------------------------------------
+// doViscousStep (source/hydro_solver.f90:217-250).  grid (16, nblk).
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_viscous(DevCfg c, const double *UP, double *U) {
+  int b = blockIdx.y;
+  int t = blockIdx.x * 256 + threadIdx.x;
+  int i = t % NC + 1, j = (t / NC) % NC + 1, k = t / (NC * NC) + 1;
+  int q = cid(i, j, k);
+  double eta = c.visc_eta;
+  for (int v = 0; v < c.nv; v++) {
+    const double *A = UP + ((size_t)b * c.nv + v) * NCELL;
+    U[((size_t)b * c.nv + v) * NCELL + q] =
+        A[q] + eta * (+A[q + 1] + A[q - 1] + A[q + NB] + A[q - NB] + A[q + NB * NB] + A[q - NB * NB] - 6.0 * A[q]);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// getTimestep (source/hydro_solver.f90:129-211): min over interior cells.
+// Positive doubles order like their bit patterns, so the global minimum is
+// an unsigned 64-bit atomicMin on the bits.  grid (16, nblk).
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_timestep(DevCfg c, const double *P, unsigned long long *dtbits) {
+  int b = blockIdx.y;
+  int t = blockIdx.x * 256 + threadIdx.x;
+  int i = t % NC + 1, j = (t / NC) % NC + 1, k = t / (NC * NC) + 1;
+  int nv = c.nv, l = c.info[b].level - 1;
+  const double *Pb = P + (size_t)b * nv * NCELL;
+  int q = cid(i, j, k);
+  double p[MAXV];
+  for (int v = 0; v < nv; v++) p[v] = Pb[v * NCELL + q];
+  double dt = 1.0e30;
+  if (c.pd.mhd && nv == 8) {
+    double cfx = cfast1(c.pd, p[4], p[0], p[5], p[6], p[7], p[5]);
+    double cfy = cfast1(c.pd, p[4], p[0], p[5], p[6], p[7], p[6]);
+    double cfz = cfast1(c.pd, p[4], p[0], p[5], p[6], p[7], p[7]);
+    dt = fmin(dt, c.dx[0][l] / (fabs(p[1]) + cfx));
+    dt = fmin(dt, c.dx[1][l] / (fabs(p[2]) + cfy));
+    dt = fmin(dt, c.dx[2][l] / (fabs(p[3]) + cfz));
+  } else {
+    double cs = sound(c.pd, p);
+    dt = fmin(dt, c.dx[0][l] / (fabs(p[1]) + cs));
+    dt = fmin(dt, c.dx[1][l] / (fabs(p[2]) + cs));
+    dt = fmin(dt, c.dx[2][l] / (fabs(p[3]) + cs));
+  }
+  // wavefront min, then one atomic per wavefront
+  for (int off = 32; off > 0; off >>= 1) dt = fmin(dt, __shfl_xor(dt, off));
+  if ((threadIdx.x & 63) == 0) atomicMin(dtbits, (unsigned long long)__double_as_longlong(dt));
+}
+
+// ---------------------------------------------------------------------------
+// cooling (source/cooling.f90:37-118, apply_cooling :285-371).  grid (16, nblk)
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_cooling(DevCfg c, double *U, double *P, double dt) {
+  int b = blockIdx.y;
+  int t = blockIdx.x * 256 + threadIdx.x;
+  int i = t % NC + 1, j = (t / NC) % NC + 1, k = t / (NC * NC) + 1;
+  int nv = c.nv, q = cid(i, j, k);
+  double *Pb = P + (size_t)b * nv * NCELL;
+  double pr[5];
+  for (int v = 0; v < 5; v++) pr[v] = Pb[v * NCELL + q];
+  double e;
+  if (cool_cell(c, dt, pr, &e)) {
+    Pb[4 * NCELL + q] = pr[4];
+    U[((size_t)b * nv + 4) * NCELL + q] = e;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host launchers
+// ---------------------------------------------------------------------------
+void plain_boundary(const DevCfg &c, double *A, int depth, hipStream_t st) {
+  hipLaunchKernelGGL(k_boundary, dim3(6 * depth, c.nblk), dim3(256), 0, st, c, A, depth);
+  if (c.nwind > 0) hipLaunchKernelGGL(k_winds, dim3(28, c.nblk), dim3(256), 0, st, c, A);
+}
+void plain_prims(const DevCfg &c, const double *A, double *P, int cells, unsigned long long *bad,
+                 hipStream_t st) {
+  hipLaunchKernelGGL(k_prims, dim3(28, c.nblk), dim3(256), 0, st, c, A, P, cells, bad);
+}
+void plain_fluxes(const DevCfg &c, const double *P, double *F, int order, hipStream_t st) {
+  hipLaunchKernelGGL(k_fluxes, dim3(3 * 17, c.nblk), dim3(256), 0, st, c, P, F, order);
+}
+void plain_upwind(const DevCfg &c, const double *U, const double *P, const double *F, double *UP, double dtp,
+                  hipStream_t st) {
+  hipLaunchKernelGGL(k_upwind, dim3(16, c.nblk), dim3(256), 0, st, c, U, P, F, UP, dtp);
+}
+void plain_viscous(const DevCfg &c, const double *UP, double *U, hipStream_t st) {
+  hipLaunchKernelGGL(k_viscous, dim3(16, c.nblk), dim3(256), 0, st, c, UP, U);
+}
+void plain_timestep(const DevCfg &c, const double *P, unsigned long long *dtbits, hipStream_t st) {
+  hipLaunchKernelGGL(k_timestep, dim3(16, c.nblk), dim3(256), 0, st, c, P, dtbits);
+}
+void plain_cooling(const DevCfg &c, double *U, double *P, double dt, hipStream_t st) {
+  hipLaunchKernelGGL(k_cooling, dim3(16, c.nblk), dim3(256), 0, st, c, U, P, dt);
+}
+size_t plain_flux_elems(int nblk, int nv) { return (size_t)nblk * 3 * nv * NIF; }
+
+}  // namespace wx

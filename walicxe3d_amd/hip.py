@@ -1,0 +1,236 @@
+"""ctypes binding of libwalicxe_hip.so (include/walicxe_hip.h) and a host-side
+mirror of the reference's integrator interface.
+
+The class `Walicxe3D` exposes the reference's routine names
+(source/hydro_solver.f90 `hydroSolver`, `getTimestep`, `doViscousStep`;
+source/godunov.f90 `Godunov`; source/boundary.f90 `boundary`;
+source/hydro_core.f90 `calcPrimsAll`, `updatePrims`; source/cooling.f90
+`cooling`) on device-resident state.  There is no CPU fallback: without the
+built library or a GPU every call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libwalicxe_hip.so")
+
+PLAIN_FP64 = 0
+FUSED_FP32 = 1
+LAYOUT_BLOCKS = 0
+LAYOUT_FORTRAN = 1
+U, UP = 0, 1
+CELLS_ALL, CELLS_PHYS, CELLS_GHOST = 0, 1, 2
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+class WxConfig(C.Structure):
+    _fields_ = [("nv", C.c_int), ("mhd", C.c_int), ("solver", C.c_int), ("limiter", C.c_int),
+                ("eight_wave", C.c_int), ("cooling", C.c_int), ("eos", C.c_int), ("bc", C.c_int * 6),
+                ("gamma", C.c_double), ("cv", C.c_double), ("cfl", C.c_double), ("visc_eta", C.c_double),
+                ("cooling_limit", C.c_double), ("mu0", C.c_double), ("mui", C.c_double),
+                ("ion_thres", C.c_double), ("d_sc", C.c_double), ("e_sc", C.c_double),
+                ("t_sc", C.c_double), ("p_sc", C.c_double), ("l_sc", C.c_double), ("v_sc", C.c_double),
+                ("ntab", C.c_int), ("tab_logT", _dp), ("tab_loglam", _dp),
+                ("nwind", C.c_int), ("winds", _dp), ("form", C.c_int)]
+
+
+# every exported symbol of include/walicxe_hip.h with its signature
+SIGNATURES = {
+    "wx_init": (C.c_int, [C.POINTER(WxConfig), C.POINTER(C.c_void_p)]),
+    "wx_finalize": (C.c_int, [C.c_void_p]),
+    "wx_last_error": (C.c_char_p, []),
+    "wx_version": (C.c_char_p, []),
+    "wx_set_mesh": (C.c_int, [C.c_void_p, _ip, C.c_int, _dp, C.c_int, _ip]),
+    "wx_upload_state": (C.c_int, [C.c_void_p, _dp, C.c_int, C.c_int, _ip]),
+    "wx_download_state": (C.c_int, [C.c_void_p, _dp, _dp, C.c_int, C.c_int, _ip]),
+    "wx_get_timestep": (C.c_int, [C.c_void_p, _dp]),
+    "wx_godunov": (C.c_int, [C.c_void_p, C.c_int, C.c_double]),
+    "wx_viscous_step": (C.c_int, [C.c_void_p]),
+    "wx_update_prims": (C.c_int, [C.c_void_p]),
+    "wx_cooling": (C.c_int, [C.c_void_p, C.c_double]),
+    "wx_step": (C.c_int, [C.c_void_p, _ip, _dp, _dp, _ip, C.c_double, C.c_double]),
+    "wx_run": (C.c_int, [C.c_void_p, C.c_int, _ip, _dp, _dp, _ip, C.c_double, C.c_double]),
+    "wx_boundary": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    "wx_calc_prims": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    "wx_fluxes_update": (C.c_int, [C.c_void_p, C.c_int, C.c_double]),
+    "wx_copy_raw": (C.c_int, [C.c_void_p, C.c_int, _dp]),
+    "wx_set_tuning": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    "wx_last_timing": (C.c_int, [C.c_void_p, _dp, _dp, _dp]),
+    "wx_time_stages": (C.c_int, [C.c_void_p, C.c_int]),
+}
+
+_lib = None
+
+
+def load():
+    """Load the library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            raise RuntimeError("libwalicxe_hip.so is not built: run `python -m walicxe3d_amd.build`")
+        L = C.CDLL(LIBPATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+class WxError(RuntimeError):
+    pass
+
+
+def _check(rc, what):
+    if rc < 0:
+        raise WxError("%s failed (%d): %s" % (what, rc, load().wx_last_error().decode()))
+    return rc
+
+
+def _p(a):
+    return a.ctypes.data_as(_dp)
+
+
+def make_config(w, form=PLAIN_FP64, table=None, winds=None):
+    """wx_config for a workload (walicxe3d_amd.workloads.Workload)."""
+    from walicxe3d_amd import workloads as W
+    c = WxConfig()
+    c.nv = w.nv
+    c.mhd = int(w.mhd)
+    c.solver = W.SOLVERS[w.solver]
+    c.limiter = W.LIMITERS[w.limiter]
+    c.eight_wave = int(w.eight_wave)
+    c.cooling = int(w.cooling)
+    c.eos = W.EOSS[w.eos]
+    for f in range(6):
+        c.bc[f] = W.BCS[w.bc[f]]
+    c.gamma, c.cv, c.cfl, c.visc_eta = w.gamma, w.cv, w.cfl, w.visc_eta
+    c.cooling_limit, c.mu0, c.mui, c.ion_thres = w.cooling_limit, w.mu0, w.mui, w.ion_thres
+    c.d_sc, c.e_sc, c.t_sc, c.p_sc, c.l_sc, c.v_sc = w.d_sc, w.p_sc, w.t_sc, w.p_sc, w.l_sc, w.v_sc
+    keep = []
+    if w.cooling:
+        lt, lam = table if table is not None else W.cooling_table()
+        lt = np.ascontiguousarray(lt, dtype=np.float64)
+        lam = np.ascontiguousarray(lam, dtype=np.float64)
+        keep += [lt, lam]
+        c.ntab, c.tab_logT, c.tab_loglam = len(lt), _p(lt), _p(lam)
+    wp = winds if winds is not None else W.wind_params(w)
+    if wp:
+        arr = np.ascontiguousarray(np.asarray(wp, dtype=np.float64).reshape(-1))
+        keep.append(arr)
+        c.nwind, c.winds = len(wp), _p(arr)
+    c.form = form
+    c._keep = keep
+    return c
+
+
+class Walicxe3D:
+    """Device-resident integrator for one mesh, with the reference's routine
+    names.  `it`, `time`, `dt` and `nextout` mirror the reference globals
+    (source/globals.f90:44-48)."""
+
+    def __init__(self, w, form=PLAIN_FP64, bids=None):
+        L = load()
+        self.w = w
+        self.form = form
+        self.cfg = make_config(w, form)
+        h = C.c_void_p()
+        _check(L.wx_init(C.byref(self.cfg), C.byref(h)), "wx_init")
+        self._h = h
+        m = w.mesh()
+        self.bids = np.ascontiguousarray(bids if bids is not None else w.blocks(), dtype=np.int32)
+        dx = np.ascontiguousarray(np.array(m.dx, dtype=np.float64).reshape(-1))
+        nroot = (C.c_int * 3)(*m.nroot)
+        _check(L.wx_set_mesh(h, nroot, m.maxlev, _p(dx), len(self.bids), self.bids.ctypes.data_as(_ip)),
+               "wx_set_mesh")
+        self.nblk = len(self.bids)
+        self.it = C.c_int(0)
+        self.dt = C.c_double(0.0)
+        self.time = C.c_double(0.0)
+        self.nextout = C.c_int(1)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load().wx_finalize(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    # ---- state -------------------------------------------------------------
+    def upload(self, Ub):
+        Ub = np.ascontiguousarray(Ub, dtype=np.float64)
+        assert Ub.shape == (self.nblk, self.w.nv, 16, 16, 16), Ub.shape
+        return _check(load().wx_upload_state(self._h, _p(Ub), LAYOUT_BLOCKS, 0, None), "wx_upload_state")
+
+    def download(self, prim=True):
+        u = np.empty((self.nblk, self.w.nv, 16, 16, 16))
+        p = np.empty_like(u) if prim else None
+        _check(load().wx_download_state(self._h, _p(u), _p(p) if prim else None, LAYOUT_BLOCKS, 0, None),
+               "wx_download_state")
+        return (u, p) if prim else u
+
+    def raw(self, which):
+        n = self.nblk * self.w.nv * 20 ** 3
+        out = np.empty(n)
+        _check(load().wx_copy_raw(self._h, which, _p(out)), "wx_copy_raw")
+        return out.reshape(self.nblk, self.w.nv, 20, 20, 20)
+
+    # ---- the reference's routines --------------------------------------------
+    def getTimestep(self):
+        """Raw CFL timestep (before the start-up ramp and output clamp)."""
+        d = C.c_double()
+        _check(load().wx_get_timestep(self._h, C.byref(d)), "wx_get_timestep")
+        return d.value
+
+    def Godunov(self, order, dt):
+        return _check(load().wx_godunov(self._h, order, dt), "wx_godunov")
+
+    def doViscousStep(self):
+        return _check(load().wx_viscous_step(self._h), "wx_viscous_step")
+
+    def updatePrims(self):
+        return _check(load().wx_update_prims(self._h), "wx_update_prims")
+
+    def cooling(self, dt):
+        return _check(load().wx_cooling(self._h, dt), "wx_cooling")
+
+    def boundary(self, depth, which=U):
+        return _check(load().wx_boundary(self._h, depth, which), "wx_boundary")
+
+    def calcPrimsAll(self, which, cells):
+        return _check(load().wx_calc_prims(self._h, which, cells), "wx_calc_prims")
+
+    def fluxes_update(self, order, dtp):
+        return _check(load().wx_fluxes_update(self._h, order, dtp), "wx_fluxes_update")
+
+    def step(self):
+        """One main-loop iteration: hydroSolver, updatePrims, cooling."""
+        return _check(load().wx_step(self._h, C.byref(self.it), C.byref(self.dt), C.byref(self.time),
+                                     C.byref(self.nextout), self.w.dtout, self.w.t_sc), "wx_step")
+
+    def run(self, nsteps):
+        return _check(load().wx_run(self._h, nsteps, C.byref(self.it), C.byref(self.dt), C.byref(self.time),
+                                    C.byref(self.nextout), self.w.dtout, self.w.t_sc), "wx_run")
+
+    def set_tuning(self, kind, value):
+        return _check(load().wx_set_tuning(self._h, kind, value), "wx_set_tuning")
+
+    def time_stages(self, reps):
+        """Average stage-kernel durations (ms) from HIP events on the
+        library's stream, fused form only."""
+        _check(load().wx_time_stages(self._h, reps), "wx_time_stages")
+        a, b, t = C.c_double(), C.c_double(), C.c_double()
+        load().wx_last_timing(self._h, C.byref(a), C.byref(b), C.byref(t))
+        return a.value, b.value
+
+    def last_run_ms_per_step(self):
+        a, b, t = C.c_double(), C.c_double(), C.c_double()
+        load().wx_last_timing(self._h, C.byref(a), C.byref(b), C.byref(t))
+        return t.value
