@@ -35,6 +35,7 @@ extern "C" {
 /* numerical forms */
 #define WX_FORM_PLAIN_FP64 0   /* one kernel per reference routine, FP64, reference order */
 #define WX_FORM_FUSED_FP32 1   /* one fused kernel per stage, FP32 bulk, FP64 accumulation */
+#define WX_FORM_FUSED_FP64 2   /* the fused kernels with FP64 bulk arithmetic (precision reference) */
 
 #define WX_LAYOUT_BLOCKS 0
 #define WX_LAYOUT_FORTRAN 1

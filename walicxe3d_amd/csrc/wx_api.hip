@@ -314,6 +314,7 @@ int update_prims(Ctx *c, long *bad) {
 }
 
 Ctx *as_ctx(void *p) { return static_cast<Ctx *>(p); }
+bool is_fused(const Ctx *c) { return c->cfg.form != WX_FORM_PLAIN_FP64; }
 
 int need_state(Ctx *c) {
   if (!c) return fail(WX_EINVAL, "null context");
@@ -339,8 +340,9 @@ int wx_init(const wx_config *cfg, void **out) {
     return fail(WX_EINVAL, "cooling needs a table of 2..512 points");
   if (cfg->nwind < 0 || cfg->nwind > MAXWIND || (cfg->nwind && !cfg->winds))
     return fail(WX_EINVAL, "up to 8 wind sources");
-  if (cfg->form != WX_FORM_PLAIN_FP64 && cfg->form != WX_FORM_FUSED_FP32) return fail(WX_EINVAL, "unknown form");
-  if (cfg->form == WX_FORM_FUSED_FP32 && cfg->solver == SOLVER_HLL1)
+  if (cfg->form != WX_FORM_PLAIN_FP64 && cfg->form != WX_FORM_FUSED_FP32 && cfg->form != WX_FORM_FUSED_FP64)
+    return fail(WX_EINVAL, "unknown form");
+  if (cfg->form != WX_FORM_PLAIN_FP64 && cfg->solver == SOLVER_HLL1)
     return fail(WX_EINVAL, "the fused form runs the two-stage scheme (SOLVER_HLL1 needs the plain form)");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(WX_EHIP, "no HIP device");
@@ -410,13 +412,14 @@ int wx_set_mesh(void *ctx, const int nroot[3], int maxlev, const double *dxyz, i
   HIPCHK(hipMalloc(&c->dU, n * sizeof(double)));
   HIPCHK(hipMalloc(&c->dP, n * sizeof(double)));
   HIPCHK(hipMemset(c->dP, 0, n * sizeof(double)));
-  if (c->cfg.form == WX_FORM_PLAIN_FP64) {
+  if (!is_fused(c)) {
     HIPCHK(hipMalloc(&c->dUP, n * sizeof(double)));
     HIPCHK(hipMemset(c->dUP, 0, n * sizeof(double)));
     HIPCHK(hipMalloc(&c->dF, plain_flux_elems(nblk, c->cfg.nv) * sizeof(double)));
   }
   fill_devcfg(c, dxyz);
-  if (c->cfg.form == WX_FORM_FUSED_FP32) {
+  if (is_fused(c)) {
+    c->fused.fp64 = c->cfg.form == WX_FORM_FUSED_FP64;
     int frc = fused_alloc(c->fused, c->dc, c->info, c->st);
     if (frc != 0) return fail(WX_ENOMEM, "fused-form allocation failed");
   }
@@ -449,7 +452,7 @@ int wx_upload_state(void *ctx, const double *u, int layout, int nbmax, const int
   hipFree(d_in);
   if (rc != WX_OK) return rc;
   c->have_state = true;
-  if (c->cfg.form == WX_FORM_FUSED_FP32) return fused_upload(c->fused, c->dc, c->dU, c->st) ? WX_EHIP : WX_OK;
+  if (is_fused(c)) return fused_upload(c->fused, c->dc, c->dU, c->st) ? WX_EHIP : WX_OK;
   long bad = 0;
   rc = update_prims(c, &bad);
   if (rc != WX_OK) return rc;
@@ -462,7 +465,7 @@ int wx_download_state(void *ctx, double *u, double *prim, int layout, int nbmax,
   if (rc != WX_OK) return rc;
   int nv = c->cfg.nv;
   size_t ni = (size_t)c->nblk * nv * NINT;
-  if (c->cfg.form == WX_FORM_FUSED_FP32) {
+  if (is_fused(c)) {
     if (fused_download(c->fused, c->dc, c->dU, c->dP, c->st)) return fail(WX_EHIP, "fused download failed");
   }
   double *d_out = nullptr;
@@ -487,7 +490,7 @@ int wx_get_timestep(void *ctx, double *dt_raw) {
   int rc = need_state(c);
   if (rc != WX_OK) return rc;
   if (!dt_raw) return fail(WX_EINVAL, "null dt");
-  if (c->cfg.form == WX_FORM_FUSED_FP32) return fused_timestep(c->fused, c->dc, c->st, dt_raw) ? WX_EHIP : WX_OK;
+  if (is_fused(c)) return fused_timestep(c->fused, c->dc, c->st, dt_raw) ? WX_EHIP : WX_OK;
   return plain_timestep_raw(c, dt_raw);
 }
 
@@ -496,7 +499,7 @@ int wx_godunov(void *ctx, int order, double dt) {
   int rc = need_state(c);
   if (rc != WX_OK) return rc;
   if (order != 1 && order != 2) return fail(WX_EINVAL, "order must be 1 or 2");
-  if (c->cfg.form == WX_FORM_FUSED_FP32) return fail(WX_EINVAL, "wx_godunov is a plain-form entry point");
+  if (is_fused(c)) return fail(WX_EINVAL, "wx_godunov is a plain-form entry point");
   return plain_godunov(c, order, dt);
 }
 
@@ -504,7 +507,7 @@ int wx_viscous_step(void *ctx) {
   Ctx *c = as_ctx(ctx);
   int rc = need_state(c);
   if (rc != WX_OK) return rc;
-  if (c->cfg.form == WX_FORM_FUSED_FP32) return fail(WX_EINVAL, "plain-form entry point");
+  if (is_fused(c)) return fail(WX_EINVAL, "plain-form entry point");
   plain_viscous(c->dc, c->dUP, c->dU, c->st);
   return launch_check("viscous");
 }
@@ -513,7 +516,7 @@ int wx_update_prims(void *ctx) {
   Ctx *c = as_ctx(ctx);
   int rc = need_state(c);
   if (rc != WX_OK) return rc;
-  if (c->cfg.form == WX_FORM_FUSED_FP32) return fail(WX_EINVAL, "plain-form entry point");
+  if (is_fused(c)) return fail(WX_EINVAL, "plain-form entry point");
   long bad = 0;
   rc = update_prims(c, &bad);
   return rc != WX_OK ? rc : (int)bad;
@@ -523,7 +526,7 @@ int wx_cooling(void *ctx, double dt) {
   Ctx *c = as_ctx(ctx);
   int rc = need_state(c);
   if (rc != WX_OK) return rc;
-  if (c->cfg.form == WX_FORM_FUSED_FP32) return fail(WX_EINVAL, "plain-form entry point");
+  if (is_fused(c)) return fail(WX_EINVAL, "plain-form entry point");
   if (!c->cfg.cooling) return WX_OK;
   plain_cooling(c->dc, c->dU, c->dP, dt, c->st);
   return launch_check("cooling");
@@ -534,7 +537,7 @@ int wx_step(void *ctx, int *it, double *dt, double *time, int *nextout, double d
   int rc = need_state(c);
   if (rc != WX_OK) return rc;
   if (!it || !dt || !time || !nextout) return fail(WX_EINVAL, "null argument");
-  if (c->cfg.form == WX_FORM_FUSED_FP32) return wx_run(ctx, 1, it, dt, time, nextout, dtout, t_sc);
+  if (is_fused(c)) return wx_run(ctx, 1, it, dt, time, nextout, dtout, t_sc);
   *it += 1;
   double dtl;
   rc = plain_timestep_raw(c, &dtl);
@@ -566,7 +569,7 @@ int wx_run(void *ctx, int nsteps, int *it, double *dt, double *time, int *nextou
   int rc = need_state(c);
   if (rc != WX_OK) return rc;
   if (!it || !dt || !time || !nextout || nsteps < 0) return fail(WX_EINVAL, "bad argument");
-  if (c->cfg.form != WX_FORM_FUSED_FP32) {
+  if (!is_fused(c)) {
     for (int s = 0; s < nsteps; s++) {
       rc = wx_step(ctx, it, dt, time, nextout, dtout, t_sc);
       if (rc < 0) return rc;
@@ -582,7 +585,7 @@ int wx_boundary(void *ctx, int depth, int which) {
   Ctx *c = as_ctx(ctx);
   int rc = need_state(c);
   if (rc != WX_OK) return rc;
-  if (c->cfg.form != WX_FORM_PLAIN_FP64) return fail(WX_EINVAL, "plain-form entry point");
+  if (is_fused(c)) return fail(WX_EINVAL, "plain-form entry point");
   if (depth != 1 && depth != 2) return fail(WX_EINVAL, "depth must be 1 or 2");
   plain_boundary(c->dc, which == WX_UP ? c->dUP : c->dU, depth, c->st);
   return launch_check("boundary");
@@ -592,7 +595,7 @@ int wx_calc_prims(void *ctx, int which, int cells) {
   Ctx *c = as_ctx(ctx);
   int rc = need_state(c);
   if (rc != WX_OK) return rc;
-  if (c->cfg.form != WX_FORM_PLAIN_FP64) return fail(WX_EINVAL, "plain-form entry point");
+  if (is_fused(c)) return fail(WX_EINVAL, "plain-form entry point");
   if (cells < 0 || cells > 2) return fail(WX_EINVAL, "bad cell range");
   plain_prims(c->dc, which == WX_UP ? c->dUP : c->dU, c->dP, cells, nullptr, c->st);
   return launch_check("calc_prims");
@@ -602,7 +605,7 @@ int wx_fluxes_update(void *ctx, int order, double dtp) {
   Ctx *c = as_ctx(ctx);
   int rc = need_state(c);
   if (rc != WX_OK) return rc;
-  if (c->cfg.form != WX_FORM_PLAIN_FP64) return fail(WX_EINVAL, "plain-form entry point");
+  if (is_fused(c)) return fail(WX_EINVAL, "plain-form entry point");
   plain_fluxes(c->dc, c->dP, c->dF, order, c->st);
   plain_upwind(c->dc, c->dU, c->dP, c->dF, c->dUP, dtp, c->st);
   return launch_check("fluxes_update");
@@ -612,7 +615,7 @@ int wx_copy_raw(void *ctx, int which, double *out) {
   Ctx *c = as_ctx(ctx);
   int rc = need_state(c);
   if (rc != WX_OK) return rc;
-  if (c->cfg.form != WX_FORM_PLAIN_FP64) return fail(WX_EINVAL, "plain-form entry point");
+  if (is_fused(c)) return fail(WX_EINVAL, "plain-form entry point");
   const double *src = which == 0 ? c->dU : which == 1 ? c->dUP : c->dP;
   size_t n = (size_t)c->nblk * c->cfg.nv * NCELL;
   HIPCHK(hipMemcpyAsync(out, src, n * sizeof(double), hipMemcpyDeviceToHost, c->st));
@@ -630,7 +633,7 @@ int wx_time_stages(void *ctx, int reps) {
   Ctx *c = as_ctx(ctx);
   int rc = need_state(c);
   if (rc != WX_OK) return rc;
-  if (c->cfg.form != WX_FORM_FUSED_FP32) return fail(WX_EINVAL, "fused-form entry point");
+  if (!is_fused(c)) return fail(WX_EINVAL, "fused-form entry point");
   if (reps < 1) return fail(WX_EINVAL, "reps must be positive");
   if (fused_time_stages(c->fused, c->dc, c->st, reps)) return fail(WX_EHIP, fused_error());
   return WX_OK;

@@ -26,7 +26,8 @@ struct StepState {
 
 struct FusedState {
   double *U = nullptr;       // [blk][nv][4096] FP64 conserved state
-  float *UP = nullptr;       // [blk][nv][4096] FP32 half-step state
+  void *UP = nullptr;        // [blk][nv][4096] half-step state (FP32, or FP64 in the fp64 variant)
+  int fp64 = 0;              // flux arithmetic and UP storage in FP64
   int *order = nullptr;      // block processing order (device)
   StepState *step = nullptr; // device
   int nblk = 0, nv = 0;

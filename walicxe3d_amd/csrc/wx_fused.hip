@@ -37,12 +37,16 @@ struct FusedArgs {
   DevCfg c;
   const double *U;   // stage 1 input (FP64) / stage 2 base
   double *Uout;      // stage 2 output (in place)
-  const float *UPin; // stage 2 input
-  float *UPout;      // stage 1 output
+  const void *UPin;  // stage 2 input (storage type ST)
+  void *UPout;       // stage 1 output
   const int *order;
   StepState *step;
   int nblk;
 };
+
+template <typename RF> __device__ __forceinline__ const Phys<RF> &phys(const DevCfg &c);
+template <> __device__ __forceinline__ const Phys<float> &phys<float>(const DevCfg &c) { return c.pf; }
+template <> __device__ __forceinline__ const Phys<double> &phys<double>(const DevCfg &c) { return c.pd; }
 
 // conserved state of cell (i,j,k) of block b: interior or face ghost,
 // then the wind sources; T is the storage type of the input array
@@ -90,26 +94,26 @@ __device__ __forceinline__ void load_cell(const FusedArgs &a, const T *W, int b,
   }
 }
 
-template <int NV>
-__device__ __forceinline__ void prims_f(const DevCfg &c, const double *u, float *p, bool fp64) {
+// primitives in the flux precision RF; fp64 = recover pressure in FP64
+template <int NV, typename RF>
+__device__ __forceinline__ void prims_r(const DevCfg &c, const double *u, RF *p, bool fp64) {
   if (fp64) {
     double pd[MAXV];
     flow2prim(c.pd, u, pd);
 #pragma unroll
-    for (int v = 0; v < NV; v++) p[v] = (float)pd[v];
+    for (int v = 0; v < NV; v++) p[v] = (RF)pd[v];
   } else {
-    float uf[MAXV];
+    RF ur[MAXV];
 #pragma unroll
-    for (int v = 0; v < NV; v++) uf[v] = (float)u[v];
-    flow2prim(c.pf, uf, p);
+    for (int v = 0; v < NV; v++) ur[v] = (RF)u[v];
+    flow2prim(phys<RF>(c), ur, p);
   }
 }
 
 // one interface flux from four x-aligned states (order 2) or two (order 1);
 // dim selects the rotation
-template <int NV, int ORD>
-__device__ __forceinline__ void iface_flux(const Phys<float> &P, int dim, float *pll, float *pl, float *pr,
-                                           float *prr, float *ff) {
+template <int NV, int ORD, typename RF>
+__device__ __forceinline__ void iface_flux(const Phys<RF> &P, int dim, RF *pll, RF *pl, RF *pr, RF *prr, RF *ff) {
   swapdim(NV, dim, pl);
   swapdim(NV, dim, pr);
   if (ORD == 2) {
@@ -122,108 +126,127 @@ __device__ __forceinline__ void iface_flux(const Phys<float> &P, int dim, float 
   swapdim(NV, dim, ff);
 }
 
+// CFL candidate of one cell in FP64 (getTimestep, source/hydro_solver.f90:150-182)
 template <int NV>
-__device__ __forceinline__ float dt_cell(const DevCfg &c, const double *pd, float dx, float dy, float dz) {
-  float p[MAXV];
-#pragma unroll
-  for (int v = 0; v < NV; v++) p[v] = (float)pd[v];
-  float r;
-  if (NV == 8 && c.pf.mhd) {
-    float cfx = cfast1(c.pf, p[4], p[0], p[5], p[6], p[7], p[5]);
-    float cfy = cfast1(c.pf, p[4], p[0], p[5], p[6], p[7], p[6]);
-    float cfz = cfast1(c.pf, p[4], p[0], p[5], p[6], p[7], p[7]);
-    r = fminf(fminf(dx / (fabsf(p[1]) + cfx), dy / (fabsf(p[2]) + cfy)), dz / (fabsf(p[3]) + cfz));
+__device__ __forceinline__ double dt_cell(const DevCfg &c, const double *p, double dx, double dy, double dz) {
+  double r;
+  if (NV == 8 && c.pd.mhd) {
+    double cfx = cfast1(c.pd, p[4], p[0], p[5], p[6], p[7], p[5]);
+    double cfy = cfast1(c.pd, p[4], p[0], p[5], p[6], p[7], p[6]);
+    double cfz = cfast1(c.pd, p[4], p[0], p[5], p[6], p[7], p[7]);
+    r = fmin(fmin(dx / (fabs(p[1]) + cfx), dy / (fabs(p[2]) + cfy)), dz / (fabs(p[3]) + cfz));
   } else {
-    float cs = sound(c.pf, p);
-    r = fminf(fminf(dx / (fabsf(p[1]) + cs), dy / (fabsf(p[2]) + cs)), dz / (fabsf(p[3]) + cs));
+    double cs = sound(c.pd, p);
+    r = fmin(fmin(dx / (fabs(p[1]) + cs), dy / (fabs(p[2]) + cs)), dz / (fabs(p[3]) + cs));
   }
   return r;
 }
 
+// stage-2 epilogue of one cell: primitives (FP64), cooling, write, dt
+template <int NV>
+__device__ __forceinline__ void finish_cell(const FusedArgs &a, int b, int i, int j, int k, double dt, double *u,
+                                            double dx, double dy, double dz, double &dtmin) {
+  const DevCfg &c = a.c;
+  double pd[MAXV];
+  flow2prim(c.pd, u, pd);
+  if (c.cooling) {
+    double e;
+    if (cool_cell(c, dt, pd, &e)) u[4] = e;
+  }
+  double *Uo = a.Uout + (size_t)b * NV * NINT + lin(i, j, k);
+#pragma unroll
+  for (int v = 0; v < NV; v++) Uo[v * NINT] = u[v];
+  dtmin = fmin(dtmin, dt_cell<NV>(c, pd, dx, dy, dz));
+}
+
 // ---------------------------------------------------------------------------
-// Stage kernel.  S2 = second stage; VISC = artificial viscosity active.
+// Stage kernel.  S2 = second stage; VISC = artificial viscosity active;
+// RF = flux/reconstruction arithmetic; ST = storage type of UP.
 // ---------------------------------------------------------------------------
-template <int NV, bool S2, bool VISC>
+template <int NV, bool S2, bool VISC, typename RF, typename ST>
 __global__ void __launch_bounds__(256) k_stage(FusedArgs a) {
   constexpr int ORD = S2 ? 2 : 1;
   constexpr int H = ORD;
   constexpr int W = NC + 2 * H;
   constexpr int NW = S2 ? 4 : 3;               // planes in the column window
-  __shared__ float sp[NV][W][W];               // primitives of the plane (cross region)
-  __shared__ float sfx[NV][NC][NC + 1];        // x fluxes [v][j-1][s]
-  __shared__ float sfy[NV][NC + 1][NC];        // y fluxes [v][s][i-1]
-  __shared__ float sun[(S2 && VISC) ? 2 : 1][(S2 && VISC) ? NV : 1][NC + 2][NC + 2];  // UP_new planes + ghost ring
-  __shared__ float sred[4];
+  constexpr bool SV = S2 && VISC;
+  __shared__ RF sp[NV][W][W];                  // primitives of the plane (cross region)
+  __shared__ RF sfx[NV][NC][NC + 1];           // x fluxes [v][j-1][s]
+  __shared__ RF sfy[NV][NC + 1][NC];           // y fluxes [v][s][i-1]
+  __shared__ RF sun[SV ? 2 : 1][SV ? NV : 1][NC + 2][NC + 2];  // UP_new planes + half-step ghost ring
+  __shared__ double sred[4];
 
   const DevCfg &c = a.c;
+  const Phys<RF> &P = phys<RF>(c);
+  const ST *UPin = static_cast<const ST *>(a.UPin);
+  ST *UPout = static_cast<ST *>(a.UPout);
   const int tid = threadIdx.x;
   const int ti = tid & 15, tj = tid >> 4;
   const int i = ti + 1, j = tj + 1;
   const double dt = a.step->dt;
   const double dtp = S2 ? dt : dt / 2.0;
-  float dtmin = 3.0e38f;
+  double dtmin = 1.0e30;
 
   for (int bi = blockIdx.x; bi < a.nblk; bi += gridDim.x) {
     const int b = a.order[bi];
     const int l = c.info[b].level - 1;
     const double dxl = c.dx[0][l], dyl = c.dx[1][l], dzl = c.dx[2][l];
     const double dtdx = dtp / dxl, dtdy = dtp / dyl, dtdz = dtp / dzl;
-    const float idx2 = (float)(1.0 / (2.0 * dxl)), idy2 = (float)(1.0 / (2.0 * dyl)), idz2 = (float)(1.0 / (2.0 * dzl));
+    const RF idx2 = (RF)(1.0 / (2.0 * dxl)), idy2 = (RF)(1.0 / (2.0 * dyl)), idz2 = (RF)(1.0 / (2.0 * dzl));
 
-    // ---- column window of primitives: win[w] = plane (k - 1 + w) at iteration k
-    float win[NW][NV];
+    // column window of primitives; at iteration k (after the shift)
+    // S2: win = planes {k-1, k, k+1, k+2};  S1: win = {k-1, k, k+1}
+    RF win[NW][NV];
     double ucol[MAXV];
-    double ughost_lo[MAXV], ughost_hi[MAXV];   // conserved UP ghosts at planes 0 and 17 (viscosity)
-    // pre-loop planes: S2: -1, 0, 1 (win[0..2] for k=0 -> shifted below); S1: 0, 1
-    {
-      const int k0 = S2 ? -1 : 0;
-      for (int w = 0; w < NW - 1; w++) {
-        int kk = k0 + w;
-        if (S2) load_cell<float, NV>(a, a.UPin, b, i, j, kk, ucol);
-        else load_cell<double, NV>(a, a.U, b, i, j, kk, ucol);
-        if (S2 && kk == 0) {
+    double ughost_lo[MAXV], ughost_hi[MAXV];   // conserved half-step ghosts at planes 0 and 17
+    RF fzp[NV];                                // z flux (k-1 | k)
+    double unew_prev[MAXV], unew_prev2[MAXV];  // UP_new at planes k-1, k-2 (viscosity)
+    // preload so that the shift at k = 0 produces the window of plane 0
+    for (int w = 1; w < NW; w++) {
+      int kk = w - 2;                          // S2: -1, 0, 1 ; S1: -1, 0
+      if (!S2 && kk < 0) continue;
+      if (S2) load_cell<ST, NV>(a, UPin, b, i, j, kk, ucol);
+      else load_cell<double, NV>(a, a.U, b, i, j, kk, ucol);
+      if (S2 && kk == 0) {
 #pragma unroll
-          for (int v = 0; v < NV; v++) ughost_lo[v] = ucol[v];
-        }
-        prims_f<NV>(c, ucol, win[w + (S2 ? 0 : 1)], !S2);
+        for (int v = 0; v < NV; v++) ughost_lo[v] = ucol[v];
       }
+      prims_r<NV, RF>(c, ucol, win[w], !S2);
     }
-    // z flux (0|1)
-    float fzp[NV];
-    {
-      float pll[NV], pl[NV], pr[NV], prr[NV];
-      if (S2) {
-        // need plane 2 as well
-        load_cell<float, NV>(a, a.UPin, b, i, j, 2, ucol);
-        prims_f<NV>(c, ucol, win[3], false);
-#pragma unroll
-        for (int v = 0; v < NV; v++) { pll[v] = win[0][v]; pl[v] = win[1][v]; pr[v] = win[2][v]; prr[v] = win[3][v]; }
-      } else {
-#pragma unroll
-        for (int v = 0; v < NV; v++) { pl[v] = win[1][v]; pr[v] = win[2][v]; }
-      }
-      iface_flux<NV, ORD>(c.pf, 2, pll, pl, pr, prr, fzp);
-    }
-    // after the pre-loop: S2 win = {-1, 0, 1, 2}; S1 win = {?, 0, 1}
-    double unew_prev[MAXV], unew_prev2[MAXV];   // UP_new at planes k-1, k-2 (stage 2)
 
-    for (int k = 1; k <= NC; k++) {
-      __syncthreads();   // previous plane's LDS readers are done
-      // shift the window: S2 -> {k-1, k, k+1, k+2}; S1 -> {k-1, k, k+1}
+    for (int k = 0; k <= NC; k++) {
+      // shift the window and load the new top plane of the column
 #pragma unroll
       for (int w = 0; w < NW - 1; w++)
 #pragma unroll
         for (int v = 0; v < NV; v++) win[w][v] = win[w + 1][v];
       {
         int kn = S2 ? k + 2 : k + 1;
-        if (S2) load_cell<float, NV>(a, a.UPin, b, i, j, kn, ucol);
+        if (S2) load_cell<ST, NV>(a, UPin, b, i, j, kn, ucol);
         else load_cell<double, NV>(a, a.U, b, i, j, kn, ucol);
         if (S2 && kn == NC + 1) {
 #pragma unroll
           for (int v = 0; v < NV; v++) ughost_hi[v] = ucol[v];
         }
-        prims_f<NV>(c, ucol, win[NW - 1], !S2);
+        prims_r<NV, RF>(c, ucol, win[NW - 1], !S2);
       }
+      // z flux (k | k+1): one code path for every plane, the bottom face included
+      RF fzc[NV];
+      {
+        RF pll[NV], pl[NV], pr[NV], prr[NV];
+#pragma unroll
+        for (int v = 0; v < NV; v++) {
+          if (S2) { pll[v] = win[0][v]; pl[v] = win[1][v]; pr[v] = win[2][v]; prr[v] = win[3][v]; }
+          else { pl[v] = win[1][v]; pr[v] = win[2][v]; }
+        }
+        iface_flux<NV, ORD, RF>(P, 2, pll, pl, pr, prr, fzc);
+      }
+      if (k == 0) {
+#pragma unroll
+        for (int v = 0; v < NV; v++) fzp[v] = fzc[v];
+        continue;
+      }
+      __syncthreads();   // previous plane's LDS readers are done
       // own primitives of plane k into LDS; halo threads add the face halo
 #pragma unroll
       for (int v = 0; v < NV; v++) sp[v][j - 1 + H][i - 1 + H] = win[1][v];
@@ -237,15 +260,15 @@ __global__ void __launch_bounds__(256) k_stage(FusedArgs a) {
         if (part == 0) { hi = side ? NC + g : 1 - g; hj = aa; }
         else { hj = side ? NC + g : 1 - g; hi = aa; }
         double uh[MAXV];
-        float ph[MAXV];
-        if (S2) load_cell<float, NV>(a, a.UPin, b, hi, hj, k, uh);
+        RF ph[MAXV];
+        if (S2) load_cell<ST, NV>(a, UPin, b, hi, hj, k, uh);
         else load_cell<double, NV>(a, a.U, b, hi, hj, k, uh);
-        prims_f<NV>(c, uh, ph, !S2);
+        prims_r<NV, RF>(c, uh, ph, !S2);
 #pragma unroll
         for (int v = 0; v < NV; v++) sp[v][hj - 1 + H][hi - 1 + H] = ph[v];
-        if (S2 && VISC && g == 1) {
+        if (SV && g == 1) {
 #pragma unroll
-          for (int v = 0; v < NV; v++) sun[k & 1][(S2 && VISC) ? v : 0][hj][hi] = (float)uh[v];
+          for (int v = 0; v < NV; v++) sun[k & 1][SV ? v : 0][hj][hi] = (RF)uh[v];
         }
       }
       __syncthreads();
@@ -255,7 +278,7 @@ __global__ void __launch_bounds__(256) k_stage(FusedArgs a) {
         int r = dim ? t - (NC + 1) * NC : t;
         int s = r % (NC + 1);     // interface (s | s+1)
         int aa = r / (NC + 1) + 1;
-        float pll[NV], pl[NV], pr[NV], prr[NV], ff[NV];
+        RF pll[NV], pl[NV], pr[NV], prr[NV], ff[NV];
 #pragma unroll
         for (int v = 0; v < NV; v++) {
           if (dim == 0) {
@@ -268,7 +291,7 @@ __global__ void __launch_bounds__(256) k_stage(FusedArgs a) {
             if (ORD == 2) { pll[v] = sp[v][s - 2 + H][aa - 1 + H]; prr[v] = sp[v][s + 1 + H][aa - 1 + H]; }
           }
         }
-        iface_flux<NV, ORD>(c.pf, dim, pll, pl, pr, prr, ff);
+        iface_flux<NV, ORD, RF>(P, dim, pll, pl, pr, prr, ff);
         if (dim == 0) {
 #pragma unroll
           for (int v = 0; v < NV; v++) sfx[v][aa - 1][s] = ff[v];
@@ -277,19 +300,8 @@ __global__ void __launch_bounds__(256) k_stage(FusedArgs a) {
           for (int v = 0; v < NV; v++) sfy[v][s][aa - 1] = ff[v];
         }
       }
-      // z flux (k | k+1)
-      float fzc[NV];
-      {
-        float pll[NV], pl[NV], pr[NV], prr[NV];
-#pragma unroll
-        for (int v = 0; v < NV; v++) {
-          if (S2) { pll[v] = win[0][v]; pl[v] = win[1][v]; pr[v] = win[2][v]; prr[v] = win[3][v]; }
-          else { pl[v] = win[1][v]; pr[v] = win[2][v]; }
-        }
-        iface_flux<NV, ORD>(c.pf, 2, pll, pl, pr, prr, fzc);
-      }
       __syncthreads();
-      // conservative update of cell (i, j, k) in FP64
+      // conservative update of cell (i, j, k), accumulated in FP64
       double unew[MAXV];
       {
         const double *Ub = a.U + (size_t)b * NV * NINT + lin(i, j, k);
@@ -307,103 +319,80 @@ __global__ void __launch_bounds__(256) k_stage(FusedArgs a) {
           unew[v] = ub[v] + dtdx * (double)(sfx[v][j - 1][i - 1] - sfx[v][j - 1][i]) +
                     dtdy * (double)(sfy[v][j - 1][i - 1] - sfy[v][j][i - 1]) + dtdz * (double)(fzp[v] - fzc[v]);
         if (NV == 8 && c.eight_wave) {
-          float divB = (sp[5][j - 1 + H][i + H] - sp[5][j - 1 + H][i - 2 + H]) * idx2 +
-                       (sp[6][j + H][i - 1 + H] - sp[6][j - 2 + H][i - 1 + H]) * idy2 +
-                       (win[2][7] - win[0][7]) * idz2;
-          const float *pp = win[1];
-          float s1 = -divB * pp[5], s2 = -divB * pp[6], s3 = -divB * pp[7];
-          float s4 = -divB * (pp[1] * pp[5] + pp[2] * pp[6] + pp[3] * pp[7]);
-          float s5 = -divB * pp[1], s6 = -divB * pp[2], s7 = -divB * pp[3];
-          unew[1] += dtp * (double)s1;
-          unew[2] += dtp * (double)s2;
-          unew[3] += dtp * (double)s3;
-          unew[4] += dtp * (double)s4;
-          unew[5] += dtp * (double)s5;
-          unew[6] += dtp * (double)s6;
-          unew[7] += dtp * (double)s7;
+          // Powell source (source/sources.f90:38-95) from the stage primitives
+          RF divB = (sp[5][j - 1 + H][i + H] - sp[5][j - 1 + H][i - 2 + H]) * idx2 +
+                    (sp[6][j + H][i - 1 + H] - sp[6][j - 2 + H][i - 1 + H]) * idy2 +
+                    (win[2][7] - win[0][7]) * idz2;
+          const RF *pp = win[1];
+          unew[1] += dtp * (double)(-divB * pp[5]);
+          unew[2] += dtp * (double)(-divB * pp[6]);
+          unew[3] += dtp * (double)(-divB * pp[7]);
+          unew[4] += dtp * (double)(-divB * (pp[1] * pp[5] + pp[2] * pp[6] + pp[3] * pp[7]));
+          unew[5] += dtp * (double)(-divB * pp[1]);
+          unew[6] += dtp * (double)(-divB * pp[2]);
+          unew[7] += dtp * (double)(-divB * pp[3]);
         }
       }
 #pragma unroll
       for (int v = 0; v < NV; v++) fzp[v] = fzc[v];
 
       if (!S2) {
-        float *UPo = a.UPout + (size_t)b * NV * NINT + lin(i, j, k);
+        ST *UPo = UPout + (size_t)b * NV * NINT + lin(i, j, k);
 #pragma unroll
-        for (int v = 0; v < NV; v++) UPo[v * NINT] = (float)unew[v];
+        for (int v = 0; v < NV; v++) UPo[v * NINT] = (ST)unew[v];
         continue;
       }
 
-      // ---- stage 2 epilogue: viscosity (delayed one plane), prims, cooling, dt
-      int ke = VISC ? k - 1 : k;      // plane finished in this iteration
-      double ufin[MAXV];
-      bool have = true;
+      // ---- stage 2 epilogue: viscosity (one plane behind), prims, cooling, dt
       if (VISC) {
 #pragma unroll
-        for (int v = 0; v < NV; v++) sun[k & 1][v][j][i] = (float)unew[v];
+        for (int v = 0; v < NV; v++) sun[k & 1][SV ? v : 0][j][i] = (RF)unew[v];
         __syncthreads();
-        if (k == 1) {
-          have = false;
-        } else {
+        if (k > 1) {
           const int pb = (k - 1) & 1;
+          double ufin[MAXV];
 #pragma unroll
           for (int v = 0; v < NV; v++) {
+            const int vv = SV ? v : 0;
             double below = (k - 2 == 0) ? ughost_lo[v] : unew_prev2[v];
-            float lap = sun[pb][v][j][i + 1] + sun[pb][v][j][i - 1] + sun[pb][v][j + 1][i] + sun[pb][v][j - 1][i];
-            double l6 = (double)lap + (unew[v] + below) - 6.0 * unew_prev[v];
-            ufin[v] = unew_prev[v] + c.visc_eta * l6;
+            double lap = (double)sun[pb][vv][j][i + 1] + (double)sun[pb][vv][j][i - 1] +
+                         (double)sun[pb][vv][j + 1][i] + (double)sun[pb][vv][j - 1][i] + below + unew[v] -
+                         6.0 * unew_prev[v];
+            ufin[v] = unew_prev[v] + c.visc_eta * lap;
           }
+          finish_cell<NV>(a, b, i, j, k - 1, dt, ufin, dxl, dyl, dzl, dtmin);
         }
 #pragma unroll
         for (int v = 0; v < NV; v++) { unew_prev2[v] = unew_prev[v]; unew_prev[v] = unew[v]; }
       } else {
-#pragma unroll
-        for (int v = 0; v < NV; v++) ufin[v] = unew[v];
-      }
-      if (have) {
-        double pd[MAXV];
-        flow2prim(c.pd, ufin, pd);
-        if (c.cooling) {
-          double e;
-          if (cool_cell(c, dt, pd, &e)) ufin[4] = e;
-        }
-        double *Uo = a.Uout + (size_t)b * NV * NINT + lin(i, j, ke);
-#pragma unroll
-        for (int v = 0; v < NV; v++) Uo[v * NINT] = ufin[v];
-        dtmin = fminf(dtmin, dt_cell<NV>(c, pd, (float)dxl, (float)dyl, (float)dzl));
+        finish_cell<NV>(a, b, i, j, k, dt, unew, dxl, dyl, dzl, dtmin);
       }
     }
-    if (S2 && VISC) {
+    if (SV) {
       // plane 16: its upper neighbour is the half-step ghost at plane 17
       __syncthreads();
       const int pb = NC & 1;
       double ufin[MAXV];
 #pragma unroll
       for (int v = 0; v < NV; v++) {
-        float lap = sun[pb][v][j][i + 1] + sun[pb][v][j][i - 1] + sun[pb][v][j + 1][i] + sun[pb][v][j - 1][i];
-        double l6 = (double)lap + (ughost_hi[v] + unew_prev2[v]) - 6.0 * unew_prev[v];
-        ufin[v] = unew_prev[v] + c.visc_eta * l6;
+        const int vv = SV ? v : 0;
+        double lap = (double)sun[pb][vv][j][i + 1] + (double)sun[pb][vv][j][i - 1] + (double)sun[pb][vv][j + 1][i] +
+                     (double)sun[pb][vv][j - 1][i] + unew_prev2[v] + ughost_hi[v] - 6.0 * unew_prev[v];
+        ufin[v] = unew_prev[v] + c.visc_eta * lap;
       }
-      double pd[MAXV];
-      flow2prim(c.pd, ufin, pd);
-      if (c.cooling) {
-        double e;
-        if (cool_cell(c, dt, pd, &e)) ufin[4] = e;
-      }
-      double *Uo = a.Uout + (size_t)b * NV * NINT + lin(i, j, NC);
-#pragma unroll
-      for (int v = 0; v < NV; v++) Uo[v * NINT] = ufin[v];
-      dtmin = fminf(dtmin, dt_cell<NV>(c, pd, (float)dxl, (float)dyl, (float)dzl));
+      finish_cell<NV>(a, b, i, j, NC, dt, ufin, dxl, dyl, dzl, dtmin);
     }
     __syncthreads();
   }
   if (S2) {
-    for (int off = 32; off > 0; off >>= 1) dtmin = fminf(dtmin, __shfl_xor(dtmin, off));
+    // exact min: wavefront shuffles, then one 64-bit atomic per workgroup on
+    // the bit pattern (positive doubles order like their bits)
+    for (int off = 32; off > 0; off >>= 1) dtmin = fmin(dtmin, __shfl_xor(dtmin, off));
     if ((tid & 63) == 0) sred[tid >> 6] = dtmin;
     __syncthreads();
     if (tid == 0) {
-      float m = fminf(fminf(sred[0], sred[1]), fminf(sred[2], sred[3]));
-      double md = (double)m;
-      atomicMin(&a.step->dtmin_bits, (unsigned long long)__double_as_longlong(md));
+      double m = fmin(fmin(sred[0], sred[1]), fmin(sred[2], sred[3]));
+      atomicMin(&a.step->dtmin_bits, (unsigned long long)__double_as_longlong(m));
     }
   }
 }
@@ -414,15 +403,15 @@ __global__ void __launch_bounds__(256) k_dt_init(FusedArgs a) {
   const DevCfg &c = a.c;
   int b = blockIdx.x;
   int l = c.info[b].level - 1;
-  float m = 3.0e38f;
+  double m = 1.0e30;
   for (int q = threadIdx.x; q < NINT; q += 256) {
     double u[MAXV], pd[MAXV];
     for (int v = 0; v < NV; v++) u[v] = a.U[((size_t)b * NV + v) * NINT + q];
     flow2prim(c.pd, u, pd);
-    m = fminf(m, dt_cell<NV>(c, pd, (float)c.dx[0][l], (float)c.dx[1][l], (float)c.dx[2][l]));
+    m = fmin(m, dt_cell<NV>(c, pd, c.dx[0][l], c.dx[1][l], c.dx[2][l]));
   }
-  for (int off = 32; off > 0; off >>= 1) m = fminf(m, __shfl_xor(m, off));
-  if ((threadIdx.x & 63) == 0) atomicMin(&a.step->dtmin_bits, (unsigned long long)__double_as_longlong((double)m));
+  for (int off = 32; off > 0; off >>= 1) m = fmin(m, __shfl_xor(m, off));
+  if ((threadIdx.x & 63) == 0) atomicMin(&a.step->dtmin_bits, (unsigned long long)__double_as_longlong(m));
 }
 
 // dt of the step: CFL, start-up ramp, output clamp
@@ -503,7 +492,7 @@ int fused_alloc(FusedState &f, const DevCfg &c, const std::vector<BlockInfo> &in
   f.nv = c.nv;
   size_t n = (size_t)c.nblk * c.nv * NINT;
   FCHK(hipMalloc(&f.U, n * sizeof(double)));
-  FCHK(hipMalloc(&f.UP, n * sizeof(float)));
+  FCHK(hipMalloc(&f.UP, n * (f.fp64 ? sizeof(double) : sizeof(float))));
   FCHK(hipMalloc(&f.order, c.nblk * sizeof(int)));
   FCHK(hipMalloc(&f.step, sizeof(StepState)));
   // processing order: Morton order; with the XCD map, workgroup w runs on
@@ -601,12 +590,10 @@ int fused_timestep(FusedState &f, const DevCfg &c, hipStream_t st, double *dt_ra
   return 0;
 }
 
-static void launch_stage(FusedState &f, const DevCfg &c, hipStream_t st, int stage) {
-  FusedArgs a = make_args(f, c);
-  int grid = f.grid > 0 ? std::min(f.grid, f.nblk) : f.nblk;
-  bool visc = c.visc_eta != 0.0;
-#define WX_LAUNCH(NVv, S2v, Vv) hipLaunchKernelGGL((k_stage<NVv, S2v, Vv>), dim3(grid), dim3(256), 0, st, a)
-  if (c.nv == 8) {
+template <typename RF, typename ST>
+static void launch_stage_t(const FusedArgs &a, int grid, int nv, bool visc, int stage, hipStream_t st) {
+#define WX_LAUNCH(NVv, S2v, Vv) hipLaunchKernelGGL((k_stage<NVv, S2v, Vv, RF, ST>), dim3(grid), dim3(256), 0, st, a)
+  if (nv == 8) {
     if (stage == 1) WX_LAUNCH(8, false, false);
     else if (visc) WX_LAUNCH(8, true, true);
     else WX_LAUNCH(8, true, false);
@@ -616,6 +603,14 @@ static void launch_stage(FusedState &f, const DevCfg &c, hipStream_t st, int sta
     else WX_LAUNCH(5, true, false);
   }
 #undef WX_LAUNCH
+}
+
+static void launch_stage(FusedState &f, const DevCfg &c, hipStream_t st, int stage) {
+  FusedArgs a = make_args(f, c);
+  int grid = f.grid > 0 ? std::min(f.grid, f.nblk) : f.nblk;
+  bool visc = c.visc_eta != 0.0;
+  if (f.fp64) launch_stage_t<double, double>(a, grid, c.nv, visc, stage, st);
+  else launch_stage_t<float, float>(a, grid, c.nv, visc, stage, st);
 }
 
 static void enqueue_step(FusedState &f, const DevCfg &c, hipStream_t st) {

@@ -20,6 +20,7 @@ LIBPATH = os.path.join(HERE, "libwalicxe_hip.so")
 
 PLAIN_FP64 = 0
 FUSED_FP32 = 1
+FUSED_FP64 = 2
 LAYOUT_BLOCKS = 0
 LAYOUT_FORTRAN = 1
 U, UP = 0, 1
