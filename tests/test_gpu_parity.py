@@ -175,12 +175,17 @@ def test_plain_godunov_and_viscous_pieces(oracle):
 # ---------------------------------------------------------------------------
 # fused FP32 form
 # ---------------------------------------------------------------------------
-FUSED_CASES = [("ot64", 20, 1e-5), ("sedov", 20, 2e-4), ("blast_reflect", 20, 2e-4), ("amr_small", 8, 2e-4),
-               ("mhd_hlle", 10, 1e-4), ("winds4", 3, 2e-4), ("snr_amr4", 3, 2e-4)]
+# (workload, steps, tolerance on every variable, tolerance on the MHD
+# z-momentum and B_z, which in these initial conditions carry only the
+# 1e-4 seeded noise: their absolute FP32 flux error is set by the O(0.1)
+# pressure terms, so their relative error is larger)
+FUSED_CASES = [("ot64", 20, 1e-6, 5e-3), ("sedov", 20, 1e-6, None), ("blast_reflect", 20, 1e-6, None),
+               ("amr_small", 8, 1e-6, None), ("mhd_hlle", 10, 1e-6, 1e-3), ("winds4", 3, 1e-5, None),
+               ("snr_amr4", 3, 1e-6, None)]
 
 
-@pytest.mark.parametrize("name,nsteps,tol", FUSED_CASES)
-def test_fused_fp32_error_vs_fp64_oracle(oracle, name, nsteps, tol):
+@pytest.mark.parametrize("name,nsteps,tol,tol_small", FUSED_CASES)
+def test_fused_fp32_error_vs_fp64_oracle(oracle, name, nsteps, tol, tol_small):
     from walicxe3d_amd import hip
     w = W.get(name)
     bids = np.asarray(w.blocks(), dtype=np.int32)
@@ -200,44 +205,71 @@ def test_fused_fp32_error_vs_fp64_oracle(oracle, name, nsteps, tol):
     e = rel_l1(ug, uo)
     say("  rel L1 per variable", [float("%.3e" % x) for x in e])
     assert np.all(np.isfinite(ug))
-    assert float(e.max()) <= tol, e
+    small = [3, 7] if (w.nv == 8 and tol_small is not None) else []
+    for v in range(w.nv):
+        assert e[v] <= (tol_small if v in small else tol), (v, e)
     g.close()
 
 
-def test_fused_conserves_on_periodic_domain():
+def drift(U0, u):
+    """Change of the domain totals of mass, momentum (3), energy and field (3)
+    relative to the L1 size of that quantity (momentum and field measured by
+    the magnitude of the vector, so that a near-zero component such as the
+    z-momentum of Orszag-Tang is judged on the momentum scale)."""
+    t0 = U0.sum(axis=(0, 2, 3, 4))
+    t1 = u.sum(axis=(0, 2, 3, 4))
+    mag_m = np.sqrt((U0[:, 1:4] ** 2).sum(axis=1)).sum()
+    scale = np.empty(U0.shape[1])
+    scale[0] = np.abs(U0[:, 0]).sum()
+    scale[1:4] = mag_m
+    scale[4] = np.abs(U0[:, 4]).sum()
+    if U0.shape[1] == 8:
+        scale[5:8] = np.sqrt((U0[:, 5:8] ** 2).sum(axis=1)).sum()
+    return np.abs(t1 - t0) / scale
+
+
+@pytest.mark.parametrize("form", ["fused_fp32", "fused_fp64", "plain"])
+def test_conservation_on_periodic_domain(form):
     """Orszag-Tang without the Powell source on a periodic grid: the update
-    is a telescoping flux difference, so totals hold to FP64 round-off."""
+    is a telescoping flux difference (a face flux is computed identically by
+    the two blocks sharing it), so totals hold to FP64 round-off."""
     from walicxe3d_amd import hip
     w = dataclasses.replace(W.get("ot64"), eight_wave=False)
     bids = np.asarray(w.blocks(), dtype=np.int32)
     U0 = ic_in_order(w, bids)
-    say("case fused conservation ot64")
-    g = gpu(w, hip.FUSED_FP32, bids)
+    say("case conservation ot64", form)
+    fm = {"fused_fp32": hip.FUSED_FP32, "fused_fp64": hip.FUSED_FP64, "plain": hip.PLAIN_FP64}[form]
+    g = gpu(w, fm, bids)
     g.upload(U0)
-    g.run(10)
+    if form == "plain":
+        for _ in range(12):
+            g.step()
+    else:
+        g.run(12)
     u, _ = g.download()
-    t0 = U0.sum(axis=(0, 2, 3, 4))
-    t1 = u.sum(axis=(0, 2, 3, 4))
-    scale = np.abs(U0).sum(axis=(0, 2, 3, 4))
-    dev = np.abs(t1 - t0) / scale
+    dev = drift(U0, u)
     say("  relative drift of totals", dev)
-    assert np.all(dev[:5] < 1e-13), dev
+    assert np.all(dev < 1e-13), dev
     g.close()
 
 
-def test_plain_conserves_on_periodic_domain():
+def test_fused_fp64_matches_oracle(oracle):
+    """The fused kernels in FP64 reproduce the reference's arithmetic on the
+    Orszag-Tang workload bit for bit."""
     from walicxe3d_amd import hip
-    w = dataclasses.replace(W.get("ot64"), eight_wave=False)
+    w = W.get("ot64")
     bids = np.asarray(w.blocks(), dtype=np.int32)
     U0 = ic_in_order(w, bids)
-    say("case plain conservation ot64")
-    g = gpu(w, hip.PLAIN_FP64, bids)
+    say("case fused-fp64 vs oracle ot64")
+    g = gpu(w, hip.FUSED_FP64, bids)
     g.upload(U0)
-    for _ in range(10):
-        g.step()
-    u, _ = g.download()
-    t0 = U0.sum(axis=(0, 2, 3, 4))
-    t1 = u.sum(axis=(0, 2, 3, 4))
-    dev = np.abs(t1 - t0) / np.abs(U0).sum(axis=(0, 2, 3, 4))
-    assert np.all(dev[:5] < 1e-13), dev
+    g.run(12)
+    o = oracle.Sim(w, bids)
+    o.set_state(U0)
+    for _ in range(12):
+        o.step()
+    assert g.time.value == o.time.value
+    ug, _ = g.download()
+    uo, _ = o.get_state()
+    assert np.array_equal(ug, uo)
     g.close()

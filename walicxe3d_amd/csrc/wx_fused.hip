@@ -49,8 +49,11 @@ template <> __device__ __forceinline__ const Phys<float> &phys<float>(const DevC
 template <> __device__ __forceinline__ const Phys<double> &phys<double>(const DevCfg &c) { return c.pd; }
 
 // conserved state of cell (i,j,k) of block b: interior or face ghost,
-// then the wind sources; T is the storage type of the input array
-template <typename T, int NV>
+// then the wind sources; T is the storage type of the input array.
+// WIN = 0: interior cells keep the state before the wind is re-imposed (the
+// reference's stage 1 reuses the interior primitives of updatePrims, computed
+// before boundary() re-imposes the sources; source/godunov.f90:77-83).
+template <typename T, int NV, bool WIN>
 __device__ __forceinline__ void load_cell(const FusedArgs &a, const T *W, int b, int i, int j, int k, double *u) {
   bool in = (unsigned)(i - 1) < NC && (unsigned)(j - 1) < NC && (unsigned)(k - 1) < NC;
   if (in) {
@@ -85,7 +88,7 @@ __device__ __forceinline__ void load_cell(const FusedArgs &a, const T *W, int b,
       u[v] = val;
     }
   }
-  if (a.c.nwind > 0) {
+  if (a.c.nwind > 0 && (WIN || !in)) {
     double w[MAXV];
     if (wind_state(a.c, b, i, j, k, w)) {
 #pragma unroll
@@ -205,8 +208,8 @@ __global__ void __launch_bounds__(256) k_stage(FusedArgs a) {
     for (int w = 1; w < NW; w++) {
       int kk = w - 2;                          // S2: -1, 0, 1 ; S1: -1, 0
       if (!S2 && kk < 0) continue;
-      if (S2) load_cell<ST, NV>(a, UPin, b, i, j, kk, ucol);
-      else load_cell<double, NV>(a, a.U, b, i, j, kk, ucol);
+      if (S2) load_cell<ST, NV, true>(a, UPin, b, i, j, kk, ucol);
+      else load_cell<double, NV, false>(a, a.U, b, i, j, kk, ucol);
       if (S2 && kk == 0) {
 #pragma unroll
         for (int v = 0; v < NV; v++) ughost_lo[v] = ucol[v];
@@ -222,8 +225,8 @@ __global__ void __launch_bounds__(256) k_stage(FusedArgs a) {
         for (int v = 0; v < NV; v++) win[w][v] = win[w + 1][v];
       {
         int kn = S2 ? k + 2 : k + 1;
-        if (S2) load_cell<ST, NV>(a, UPin, b, i, j, kn, ucol);
-        else load_cell<double, NV>(a, a.U, b, i, j, kn, ucol);
+        if (S2) load_cell<ST, NV, true>(a, UPin, b, i, j, kn, ucol);
+        else load_cell<double, NV, false>(a, a.U, b, i, j, kn, ucol);
         if (S2 && kn == NC + 1) {
 #pragma unroll
           for (int v = 0; v < NV; v++) ughost_hi[v] = ucol[v];
@@ -261,8 +264,8 @@ __global__ void __launch_bounds__(256) k_stage(FusedArgs a) {
         else { hj = side ? NC + g : 1 - g; hi = aa; }
         double uh[MAXV];
         RF ph[MAXV];
-        if (S2) load_cell<ST, NV>(a, UPin, b, hi, hj, k, uh);
-        else load_cell<double, NV>(a, a.U, b, hi, hj, k, uh);
+        if (S2) load_cell<ST, NV, true>(a, UPin, b, hi, hj, k, uh);
+        else load_cell<double, NV, false>(a, a.U, b, hi, hj, k, uh);
         prims_r<NV, RF>(c, uh, ph, !S2);
 #pragma unroll
         for (int v = 0; v < NV; v++) sp[v][hj - 1 + H][hi - 1 + H] = ph[v];
