@@ -92,6 +92,11 @@ int wx_set_mesh(void *ctx, const int nroot[3], int maxlev, const double *dxyz, i
    pass nbmax = nbMaxProc and local_blocks = localBlocks; slots whose bID is
    not in the mesh are ignored. */
 int wx_upload_state(void *ctx, const double *u, int layout, int nbmax, const int *local_blocks);
+/* Overwrite the interior primitives (plain form).  The reference's stage 1
+   reads the interior PRIM left by updatePrims and cooling, whose pressure
+   update (source/cooling.f90:361-363) is not flow2prim of the cooled U; a
+   host that runs cooling itself passes its PRIM here after the upload. */
+int wx_upload_prims(void *ctx, const double *prim, int layout, int nbmax, const int *local_blocks);
 /* Download interior U and PRIM (either may be NULL). */
 int wx_download_state(void *ctx, double *u, double *prim, int layout, int nbmax,
                       const int *local_blocks);

@@ -103,9 +103,26 @@ def parameters_text(w):
     return text
 
 
-def build(name):
+FORTRAN_LAYER = os.path.join(ROOT, "walicxe3d_amd", "fortran")
+GPU_LIBDIR = os.path.join(ROOT, "walicxe3d_amd")
+
+# GPU drop-in variant: the same build with the reference's module
+# `hydro_solver` replaced by this repository's GPU implementation
+# (walicxe3d_amd/fortran/hydro_solver_gpu.f90) linked to libwalicxe_hip.so.
+def gpu_sources():
+    out = []
+    for where, fn in SOURCES:
+        if fn == "hydro_solver.f90":
+            out.append(("gpu", "walicxe_hip.f90"))
+            out.append(("gpu", "hydro_solver_gpu.f90"))
+        else:
+            out.append((where, fn))
+    return out
+
+
+def build(name, gpu=False):
     w = workloads.get(name)
-    bdir = os.path.join(OUT, name)
+    bdir = os.path.join(OUT, name + ("_gpu" if gpu else ""))
     os.makedirs(bdir, exist_ok=True)
     with open(os.path.join(bdir, "parameters.f90"), "w") as f:
         f.write(parameters_text(w))
@@ -116,8 +133,13 @@ def build(name):
     if w.mhd:
         flags.append("-DBFIELD")
     objs = []
-    for where, fn in SOURCES:
-        src = os.path.join(REF, "source", fn) if where == "ref" else os.path.join(bdir, fn)
+    for where, fn in (gpu_sources() if gpu else SOURCES):
+        if where == "ref":
+            src = os.path.join(REF, "source", fn)
+        elif where == "gpu":
+            src = os.path.join(FORTRAN_LAYER, fn)
+        else:
+            src = os.path.join(bdir, fn)
         obj = os.path.join(bdir, fn.replace(".f90", ".o"))
         cmd = [FC] + flags + ["-c", src, "-o", obj]
         r = subprocess.run(cmd, capture_output=True, text=True)
@@ -128,6 +150,8 @@ def build(name):
     exe = os.path.join(bdir, "harness")
     cmd = [FC] + objs + ["-o", exe, "-L" + MPI_LIB, "-lmpifort", "-lmpi",
                          "-Wl,-rpath," + MPI_LIB]
+    if gpu:
+        cmd += ["-L" + GPU_LIBDIR, "-lwalicxe_hip", "-Wl,-rpath,$ORIGIN/../../../walicxe3d_amd"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(" ".join(cmd) + "\n" + r.stdout + r.stderr)
@@ -139,10 +163,21 @@ def main(argv):
     if not os.path.isdir(os.path.join(REF, "source")):
         print("reference sources not present; nothing to build")
         return 0
+    gpu = "--gpu" in argv
+    argv = [a for a in argv if not a.startswith("--")]
     names = argv or workloads.oracle_builds()
-    for n in names:
-        print("building", n, "->", build(n))
+    jobs = [(n, False) for n in names]
+    if gpu or not argv:
+        # drop-in builds used by tests/test_gpu_fortran.py
+        jobs += [(n, True) for n in (argv or DROPIN_WORKLOADS)]
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(max_workers=4) as ex:
+        for (n, g), exe in zip(jobs, ex.map(lambda j: build(*j), jobs)):
+            print("building", n, "(gpu drop-in)" if g else "", "->", exe)
     return 0
+
+
+DROPIN_WORKLOADS = ["sedov", "ot64", "amr_small", "blast_reflect"]
 
 
 if __name__ == "__main__":

@@ -55,10 +55,10 @@ def read_state(path, nblk, nv_ref):
     return a.reshape(nblk, nv_ref, 16, 16, 16)
 
 
-def run(w, nsteps, dumps, workdir=None, unit_op=None, unit_in=None, timeout=3600, tstop=-1.0):
+def run(w, nsteps, dumps, workdir=None, unit_op=None, unit_in=None, timeout=3600, tstop=-1.0, variant=""):
     """Run the harness; returns dict with topo, steps (it, dt, time, tout),
     and states {step: (U, PRIM)} with the passive scalar stripped."""
-    exe = harness_path(w.name)
+    exe = harness_path(w.name + variant)
     if not os.path.exists(exe):
         raise FileNotFoundError(exe)
     own = workdir is None

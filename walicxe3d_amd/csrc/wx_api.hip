@@ -459,6 +459,35 @@ int wx_upload_state(void *ctx, const double *u, int layout, int nbmax, const int
   return (int)bad;
 }
 
+int wx_upload_prims(void *ctx, const double *p, int layout, int nbmax, const int *lb) {
+  Ctx *c = as_ctx(ctx);
+  int rc = need_state(c);
+  if (rc != WX_OK) return rc;
+  if (!p) return fail(WX_EINVAL, "null argument");
+  if (is_fused(c)) return fail(WX_EINVAL, "plain-form entry point");
+  int nv = c->cfg.nv;
+  size_t ni = (size_t)c->nblk * nv * NINT;
+  std::vector<double> tmp;
+  const double *src = p;
+  if (layout == WX_LAYOUT_FORTRAN) {
+    if (!lb || nbmax < 1) return fail(WX_EINVAL, "Fortran layout needs nbmax and localBlocks");
+    tmp.assign(ni, 0.0);
+    to_fortran_or_back(c, p, tmp.data(), true, nbmax, lb);
+    src = tmp.data();
+  } else if (layout != WX_LAYOUT_BLOCKS) {
+    return fail(WX_EINVAL, "unknown layout");
+  }
+  double *d_in = nullptr;
+  HIPCHK(hipMalloc(&d_in, ni * sizeof(double)));
+  HIPCHK(hipMemcpyAsync(d_in, src, ni * sizeof(double), hipMemcpyHostToDevice, c->st));
+  size_t total = (size_t)c->nblk * nv * NCELL;
+  hipLaunchKernelGGL(k_scatter, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, c->st, nv, c->nblk, d_in, c->dP, 0);
+  rc = launch_check("upload_prims");
+  HIPCHK(hipStreamSynchronize(c->st));
+  hipFree(d_in);
+  return rc;
+}
+
 int wx_download_state(void *ctx, double *u, double *prim, int layout, int nbmax, const int *lb) {
   Ctx *c = as_ctx(ctx);
   int rc = need_state(c);

@@ -319,8 +319,11 @@ __global__ void __launch_bounds__(256) k_stage(FusedArgs a) {
         }
 #pragma unroll
         for (int v = 0; v < NV; v++)
-          unew[v] = ub[v] + dtdx * (double)(sfx[v][j - 1][i - 1] - sfx[v][j - 1][i]) +
-                    dtdy * (double)(sfy[v][j - 1][i - 1] - sfy[v][j][i - 1]) + dtdz * (double)(fzp[v] - fzc[v]);
+          // flux differences formed in FP64 (exact for FP32 operands), so
+          // the sum over cells telescopes
+          unew[v] = ub[v] + dtdx * ((double)sfx[v][j - 1][i - 1] - (double)sfx[v][j - 1][i]) +
+                    dtdy * ((double)sfy[v][j - 1][i - 1] - (double)sfy[v][j][i - 1]) +
+                    dtdz * ((double)fzp[v] - (double)fzc[v]);
         if (NV == 8 && c.eight_wave) {
           // Powell source (source/sources.f90:38-95) from the stage primitives
           RF divB = (sp[5][j - 1 + H][i + H] - sp[5][j - 1 + H][i - 2 + H]) * idx2 +

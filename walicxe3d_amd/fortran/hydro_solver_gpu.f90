@@ -1,0 +1,243 @@
+!===============================================================================
+! Drop-in GPU replacement of Walicxe3D's module `hydro_solver`
+! (source/hydro_solver.f90): same module name, same public subroutines and
+! interfaces -
+!   hydroSolver()          source/hydro_solver.f90:39-121
+!   getTimestep(dt_glob)   source/hydro_solver.f90:129-211
+!   doViscousStep()        source/hydro_solver.f90:217-250
+! A Walicxe3D build links this file (and walicxe_hip.f90) instead of
+! source/hydro_solver.f90 plus libwalicxe_hip.so; the rest of the code is
+! unchanged.  The integrator then runs on the GPU (the plain FP64 form, whose
+! results match the CPU path bit for bit):
+!   - the GPU context is created on the first call from the `parameters`
+!     module and the cooling table in `globals`;
+!   - the block list is (re)sent whenever localBlocks changes (AMR);
+!   - each call uploads U and the interior PRIM left by updatePrims/cooling,
+!     runs getTimestep/Godunov/doViscousStep on the device and downloads U,
+!     so updatePrims, cooling, admesh and output keep working on the host
+!     arrays as before.
+! Supported: npassive = 0 on the device (passive scalars, if any, are left
+! untouched), SOLVER_HLL1/HLL/HLLC/HLLE/HLLD, COOL_NONE/COOL_TABLE, one MPI
+! rank per GPU.  userBoundary effects other than periodic faces (handled by
+! the library) must be registered through hydro_gpu_set_winds.
+!===============================================================================
+module hydro_solver
+
+  use iso_c_binding
+  use walicxe_hip
+  implicit none
+
+  private
+  public :: hydroSolver, getTimestep, doViscousStep, hydro_gpu_set_winds, hydro_gpu_release
+
+  type(c_ptr), save :: ctx = c_null_ptr
+  integer(c_int), allocatable, target, save :: mesh_bids(:)
+  integer(c_int), allocatable, target, save :: lbuf(:)
+  real(c_double), allocatable, target, save :: ubuf(:,:,:,:,:)
+  real(c_double), allocatable, target, save :: tab_T(:), tab_C(:)
+  real(c_double), allocatable, target, save :: wind_rec(:)
+  integer, save :: nwind = 0
+
+contains
+
+  !> Spherical wind sources re-imposed at every boundary call
+  !! (source/winds.f90:123): 8 values per source, xc,yc,zc,radius,mdot,vinf,
+  !! temp,mu in cgs.  Call before the first hydroSolver.
+  subroutine hydro_gpu_set_winds(n, rec)
+    integer, intent(in) :: n
+    real(c_double), intent(in) :: rec(8, n)
+    if (allocated(wind_rec)) deallocate(wind_rec)
+    allocate(wind_rec(8*max(n,1)))
+    if (n > 0) wind_rec(1:8*n) = reshape(rec, [8*n])
+    nwind = n
+  end subroutine hydro_gpu_set_winds
+
+  subroutine hydro_gpu_release()
+    integer(c_int) :: rc
+    if (c_associated(ctx)) rc = wx_finalize(ctx)
+    ctx = c_null_ptr
+    if (allocated(mesh_bids)) deallocate(mesh_bids)
+  end subroutine hydro_gpu_release
+
+  subroutine check(rc, what)
+    use clean_quit, only : clean_abort
+    use globals, only : logu
+    integer(c_int), intent(in) :: rc
+    character(*), intent(in) :: what
+    if (rc < 0) then
+      write(logu,'(a,a,a,i0,a,a)') "GPU call ", what, " failed (", rc, "): ", trim(wx_error_text())
+      call clean_abort (255)
+    end if
+  end subroutine check
+
+  !> Context and mesh for the current block list.
+  subroutine ensure_context()
+    use parameters
+    use globals
+    type(wx_config), target :: cfg
+    integer :: nb, nact, l
+    integer(c_int), allocatable :: bids(:)
+    real(c_double), allocatable :: dxyz(:)
+    integer(c_int) :: rc
+
+    if (.not. c_associated(ctx)) then
+      cfg%nv = neqhydro + neqmhd
+      cfg%mhd = merge(1, 0, mhd)
+      cfg%solver = solver_type
+      cfg%limiter = limiter_type
+      cfg%eight_wave = merge(1, 0, eight_wave)
+      cfg%cooling = merge(1, 0, cooling_type == COOL_TABLE)
+      cfg%eos = eos_type
+      cfg%bc = [bc_left, bc_right, bc_front, bc_back, bc_bottom, bc_top]
+      cfg%gamma = gamma
+      cfg%cv = cv
+      cfg%cfl = CFL
+      cfg%visc_eta = visc_eta
+      cfg%cooling_limit = cooling_limit
+      cfg%mu0 = mu0
+      cfg%mui = mui
+      cfg%ion_thres = ion_thres
+      cfg%d_sc = d_sc
+      cfg%e_sc = e_sc
+      cfg%t_sc = t_sc
+      cfg%p_sc = p_sc
+      cfg%l_sc = l_sc
+      cfg%v_sc = v_sc
+      cfg%ntab = 0
+      if (cooling_type == COOL_TABLE) then
+        allocate(tab_T(nptsT), tab_C(nptsT))
+        tab_T(:) = cooltable(1, 1:nptsT)
+        tab_C(:) = cooltable(2, 1:nptsT)
+        cfg%ntab = nptsT
+        cfg%tab_logT = c_loc(tab_T)
+        cfg%tab_loglam = c_loc(tab_C)
+      else if (cooling_type /= COOL_NONE) then
+        call check(-1_c_int, "init (only COOL_NONE and COOL_TABLE run on the GPU)")
+      end if
+      cfg%nwind = nwind
+      if (nwind > 0) cfg%winds = c_loc(wind_rec)
+      cfg%form = WX_FORM_PLAIN_FP64
+      call check(wx_init(cfg, ctx), "wx_init")
+      allocate(ubuf(nbMaxProc, neqhydro + neqmhd, nxmin:nxmax, nymin:nymax, nzmin:nzmax))
+      allocate(lbuf(nbMaxProc))
+    end if
+
+    nact = count(localBlocks(1:nbMaxProc) /= -1)
+    allocate(bids(nact))
+    nact = 0
+    do nb = 1, nbMaxProc
+      if (localBlocks(nb) /= -1) then
+        nact = nact + 1
+        bids(nact) = localBlocks(nb)
+      end if
+    end do
+    if (allocated(mesh_bids)) then
+      if (size(mesh_bids) == nact) then
+        if (all(mesh_bids == bids)) return
+      end if
+      deallocate(mesh_bids)
+    end if
+    allocate(mesh_bids(nact))
+    mesh_bids(:) = bids(:)
+    allocate(dxyz(3*maxlev))
+    do l = 1, maxlev
+      dxyz(l) = dx(l)
+      dxyz(maxlev + l) = dy(l)
+      dxyz(2*maxlev + l) = dz(l)
+    end do
+    rc = wx_set_mesh(ctx, [int(nbrootx,c_int), int(nbrooty,c_int), int(nbrootz,c_int)], &
+                     int(maxlev,c_int), dxyz, int(nact,c_int), mesh_bids)
+    call check(rc, "wx_set_mesh")
+  end subroutine ensure_context
+
+  !> U and the interior PRIM of the host arrays to the device.
+  subroutine to_device()
+    use parameters
+    use globals
+    integer :: nvg
+    nvg = neqhydro + neqmhd
+    lbuf(:) = localBlocks(1:nbMaxProc)
+    ubuf(:,:,:,:,:) = U(:, 1:nvg, :, :, :)
+    call check(wx_upload_state(ctx, c_loc(ubuf), WX_LAYOUT_FORTRAN, int(nbMaxProc,c_int), c_loc(lbuf)), &
+               "wx_upload_state")
+    ubuf(:,:,:,:,:) = PRIM(:, 1:nvg, :, :, :)
+    call check(wx_upload_prims(ctx, c_loc(ubuf), WX_LAYOUT_FORTRAN, int(nbMaxProc,c_int), c_loc(lbuf)), &
+               "wx_upload_prims")
+  end subroutine to_device
+
+  !> Device U (interior) back into the host array.
+  subroutine to_host()
+    use parameters
+    use globals
+    integer :: nvg
+    nvg = neqhydro + neqmhd
+    ubuf(:,:,:,:,:) = U(:, 1:nvg, :, :, :)
+    call check(wx_download_state(ctx, c_loc(ubuf), c_null_ptr, WX_LAYOUT_FORTRAN, int(nbMaxProc,c_int), &
+               c_loc(lbuf)), "wx_download_state")
+    U(:, 1:nvg, :, :, :) = ubuf(:,:,:,:,:)
+  end subroutine to_host
+
+  !> High-level wrapper for the numerical integrator (GPU).
+  subroutine hydroSolver ()
+    use parameters
+    use globals
+    use clean_quit, only : clean_abort
+    real :: dt_new
+    integer(c_int) :: order
+
+    if (solver_type == SOLVER_LAX) then
+      write(logu,'(a)') "SOLVER_LAX does not run on the GPU"
+      call clean_abort (ERROR_GENERIC)
+    end if
+    call ensure_context()
+    call to_device()
+
+    call getTimestep (dt_new)
+    if ((dt.gt.0.0).and.(abs(dt-dt_new)/dt.gt.50)) then
+      write(logu,'(a)') "Abnormal timestep change!"
+      write(logu,'(a,es12.5)') "Previous timestep:", dt
+      write(logu,'(a,es12.5)') "New timestep:     ", dt_new
+    end if
+    dt = dt_new
+
+    order = 2
+    if (solver_type == SOLVER_HLL1) order = 1
+    call check(wx_godunov(ctx, order, dt), "wx_godunov")
+    call doViscousStep()
+    call to_host()
+
+  end subroutine hydroSolver
+
+  !> Global timestep: CFL minimum on the device, then the start-up ramp and
+  !! the output clamp on the host as in the reference.
+  subroutine getTimestep (dt_glob)
+    use parameters
+    use globals
+    real, intent(out) :: dt_glob
+    real(c_double) :: dt_raw
+    real :: dt_loc
+
+    call check(wx_get_timestep(ctx, dt_raw), "wx_get_timestep")
+    dt_loc = dt_raw
+    if (it <= 10) then
+      dt_loc = (2.**(-real(11-it)))*dt_loc
+    end if
+#ifdef MPIP
+    call MPI_ALLREDUCE(dt_loc, dt_glob, 1, mpi_real_kind, mpi_min, mpi_comm_world, ierr)
+#else
+    dt_glob = dt_loc
+#endif
+    if ((time + dt_glob) >= nextout*dtout/t_sc) then
+      dt_glob = nextout*dtout/t_sc - time
+      dumpout = .true.
+    end if
+  end subroutine getTimestep
+
+  !> Advances the time and applies the artificial viscosity (device).
+  subroutine doViscousStep ()
+    use globals, only : time, dt
+    time = time + dt
+    call check(wx_viscous_step(ctx), "wx_viscous_step")
+  end subroutine doViscousStep
+
+end module hydro_solver

@@ -49,6 +49,7 @@ SIGNATURES = {
     "wx_version": (C.c_char_p, []),
     "wx_set_mesh": (C.c_int, [C.c_void_p, _ip, C.c_int, _dp, C.c_int, _ip]),
     "wx_upload_state": (C.c_int, [C.c_void_p, _dp, C.c_int, C.c_int, _ip]),
+    "wx_upload_prims": (C.c_int, [C.c_void_p, _dp, C.c_int, C.c_int, _ip]),
     "wx_download_state": (C.c_int, [C.c_void_p, _dp, _dp, C.c_int, C.c_int, _ip]),
     "wx_get_timestep": (C.c_int, [C.c_void_p, _dp]),
     "wx_godunov": (C.c_int, [C.c_void_p, C.c_int, C.c_double]),
