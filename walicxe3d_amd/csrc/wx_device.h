@@ -65,49 +65,57 @@ __host__ __device__ inline bool ghost_source(const BlockInfo *info, const int *b
   face_axes(f, n, t1, t2);
   bool low = (f & 1) == 0;
   s.flip = -1;
+  // source coordinates along the normal (cn) and the two transverse axes
+  int cn, c1, c2;
+  bool ok = true;
   switch (F.type) {
     case NEIGH_SAME:
       s.kind = 0;
       s.blk = F.src[0];
-      s.c[n] = low ? (NC + 1 - g) : g;
-      s.c[t1] = a1;
-      s.c[t2] = a2;
-      return true;
+      cn = low ? (NC + 1 - g) : g;
+      c1 = a1;
+      c2 = a2;
+      break;
     case NEIGH_COARSER:
       s.kind = 0;
       s.blk = F.src[0];
-      s.c[n] = low ? NC : 1;
-      s.c[t1] = (a1 + 1) / 2 + B.sib[t1] * (NC / 2);
-      s.c[t2] = (a2 + 1) / 2 + B.sib[t2] * (NC / 2);
-      return true;
+      cn = low ? NC : 1;
+      c1 = (a1 + 1) / 2 + B.sib[t1] * (NC / 2);
+      c2 = (a2 + 1) / 2 + B.sib[t2] * (NC / 2);
+      break;
     case NEIGH_FINER: {
       int q1 = a1 > NC / 2, q2 = a2 > NC / 2;
       s.kind = 1;
       s.blk = F.src[q1 + 2 * q2];
-      s.c[n] = low ? (NC + 1 - 2 * g) : (2 * g - 1);
-      s.c[t1] = 2 * (a1 - q1 * (NC / 2)) - 1;
-      s.c[t2] = 2 * (a2 - q2 * (NC / 2)) - 1;
-      return true;
+      cn = low ? (NC + 1 - 2 * g) : (2 * g - 1);
+      c1 = 2 * (a1 - q1 * (NC / 2)) - 1;
+      c2 = 2 * (a2 - q2 * (NC / 2)) - 1;
+      break;
     }
     default: {  // domain boundary
       int t = bc[f];
       s.blk = b;
-      s.c[t1] = a1;
-      s.c[t2] = a2;
+      c1 = a1;
+      c2 = a2;
+      s.kind = 0;
       if (t == BC_REFLECTIVE) {
         s.kind = 2;
-        s.c[n] = low ? g : (NC + 1 - g);
+        cn = low ? g : (NC + 1 - g);
         s.flip = 1 + n;
-        return true;
+      } else if (t == BC_FREE_FLOW) {
+        cn = low ? 1 : (NC + 1 - g);
+      } else {
+        cn = 1;
+        ok = false;
       }
-      if (t == BC_FREE_FLOW) {
-        s.kind = 0;
-        s.c[n] = low ? 1 : (NC + 1 - g);
-        return true;
-      }
-      return false;
+      break;
     }
   }
+  // scatter (cn, c1, c2) to x, y, z without run-time array indexing
+  s.c[0] = n == 0 ? cn : c1;
+  s.c[1] = n == 0 ? c1 : (n == 1 ? cn : c2);
+  s.c[2] = n == 2 ? cn : c2;
+  return ok;
 }
 
 // Value of variable v of a face ghost from its source, reading an array

@@ -39,6 +39,8 @@ struct FusedState {
   double t_s1 = 0, t_s2 = 0, t_tot = 0;
   int timing_runs = 0;
   std::vector<int> host_order;
+  std::vector<BlockInfo> host_info;
+  int maxlev = 0;
 };
 
 int fused_alloc(FusedState &f, const DevCfg &c, const std::vector<BlockInfo> &info, hipStream_t st);

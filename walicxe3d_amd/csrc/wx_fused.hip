@@ -48,6 +48,32 @@ template <typename RF> __device__ __forceinline__ const Phys<RF> &phys(const Dev
 template <> __device__ __forceinline__ const Phys<float> &phys<float>(const DevCfg &c) { return c.pf; }
 template <> __device__ __forceinline__ const Phys<double> &phys<double>(const DevCfg &c) { return c.pd; }
 
+// Riemann solver chosen at compile time (one solver's code per kernel)
+template <int SOLVER, typename RF>
+__device__ __forceinline__ void riemann_t(const Phys<RF> &P, const RF *pl, const RF *pr, RF *ff) {
+  if constexpr (SOLVER == SOLVER_HLLC) primfhllc(P, pl, pr, ff);
+  else if constexpr (SOLVER == SOLVER_HLLD) primfhlld(P, pl, pr, ff);
+  else if constexpr (SOLVER == SOLVER_HLLE) primfhlle(P, pl, pr, ff);
+  else primfhll(P, pl, pr, ff);
+}
+
+// exchange of the x components with those along d (0 x, 1 y, 2 z) of
+// velocity and field, written with selects so that the arrays stay in
+// registers when d is not a compile-time constant (swapxy/swapxz,
+// source/hydro_core.f90:448-496)
+template <int NV, typename RF> __device__ __forceinline__ void rot(int d, RF *v) {
+  RF a = v[1], b = v[2], c = v[3];
+  v[1] = d == 0 ? a : (d == 1 ? b : c);
+  v[2] = d == 1 ? a : b;
+  v[3] = d == 2 ? a : c;
+  if (NV == 8) {
+    RF e = v[5], f = v[6], g = v[7];
+    v[5] = d == 0 ? e : (d == 1 ? f : g);
+    v[6] = d == 1 ? e : f;
+    v[7] = d == 2 ? e : g;
+  }
+}
+
 // conserved state of cell (i,j,k) of block b: interior or face ghost,
 // then the wind sources; T is the storage type of the input array.
 // WIN = 0: interior cells keep the state before the wind is re-imposed (the
@@ -70,22 +96,32 @@ __device__ __forceinline__ void load_cell(const FusedArgs &a, const T *W, int b,
     else { f = TOP; g = k - NC; a1 = i; a2 = j; }
     GhostSrc s;
     ghost_source(a.c.info, a.c.bc, b, f, g, a1, a2, s);
+    const int ci = s.c[0], cj = s.c[1], ck = s.c[2];
     const T *S = W + (size_t)s.blk * NV * NINT;
+    if (s.kind == 1) {
+      const int q0 = lin(ci, cj, ck);
 #pragma unroll
-    for (int v = 0; v < NV; v++) {
-      const T *Sv = S + v * NINT;
-      double val;
-      if (s.kind == 1) {
+      for (int v = 0; v < NV; v++) {
+        const T *Sv = S + v * NINT + q0;
+        // 2x2x2 average in the reference's summation order (x, then y, then z fastest)
         double acc = 0.0;
-        for (int dx = 0; dx < 2; dx++)
-          for (int dy = 0; dy < 2; dy++)
-            for (int dz = 0; dz < 2; dz++) acc = acc + (double)Sv[lin(s.c[0] + dx, s.c[1] + dy, s.c[2] + dz)] / 8.0;
-        val = acc;
-      } else {
-        val = (double)Sv[lin(s.c[0], s.c[1], s.c[2])];
-        if (s.kind == 2 && v == s.flip) val = -val;
+        acc = acc + (double)Sv[0] / 8.0;
+        acc = acc + (double)Sv[NC * NC] / 8.0;
+        acc = acc + (double)Sv[NC] / 8.0;
+        acc = acc + (double)Sv[NC + NC * NC] / 8.0;
+        acc = acc + (double)Sv[1] / 8.0;
+        acc = acc + (double)Sv[1 + NC * NC] / 8.0;
+        acc = acc + (double)Sv[1 + NC] / 8.0;
+        acc = acc + (double)Sv[1 + NC + NC * NC] / 8.0;
+        u[v] = acc;
       }
-      u[v] = val;
+    } else {
+      const int q0 = lin(ci, cj, ck);
+#pragma unroll
+      for (int v = 0; v < NV; v++) {
+        double val = (double)S[v * NINT + q0];
+        u[v] = (s.kind == 2 && v == s.flip) ? -val : val;
+      }
     }
   }
   if (a.c.nwind > 0 && (WIN || !in)) {
@@ -111,22 +147,6 @@ __device__ __forceinline__ void prims_r(const DevCfg &c, const double *u, RF *p,
     for (int v = 0; v < NV; v++) ur[v] = (RF)u[v];
     flow2prim(phys<RF>(c), ur, p);
   }
-}
-
-// one interface flux from four x-aligned states (order 2) or two (order 1);
-// dim selects the rotation
-template <int NV, int ORD, typename RF>
-__device__ __forceinline__ void iface_flux(const Phys<RF> &P, int dim, RF *pll, RF *pl, RF *pr, RF *prr, RF *ff) {
-  swapdim(NV, dim, pl);
-  swapdim(NV, dim, pr);
-  if (ORD == 2) {
-    swapdim(NV, dim, pll);
-    swapdim(NV, dim, prr);
-#pragma unroll
-    for (int v = 0; v < NV; v++) limit1(pll[v], pl[v], pr[v], prr[v], P.limiter);
-  }
-  riemann(P, pl, pr, ff);
-  swapdim(NV, dim, ff);
 }
 
 // CFL candidate of one cell in FP64 (getTimestep, source/hydro_solver.f90:150-182)
@@ -164,18 +184,20 @@ __device__ __forceinline__ void finish_cell(const FusedArgs &a, int b, int i, in
 
 // ---------------------------------------------------------------------------
 // Stage kernel.  S2 = second stage; VISC = artificial viscosity active;
-// RF = flux/reconstruction arithmetic; ST = storage type of UP.
+// RF = flux/reconstruction arithmetic; ST = storage type of UP; SOLVER =
+// Riemann solver.  Launch bound: 256 threads, at least two waves per SIMD.
 // ---------------------------------------------------------------------------
-template <int NV, bool S2, bool VISC, typename RF, typename ST>
-__global__ void __launch_bounds__(256) k_stage(FusedArgs a) {
+template <int NV, bool S2, bool VISC, typename RF, typename ST, int SOLVER>
+__global__ void __launch_bounds__(256, 2) k_stage(FusedArgs a) {
   constexpr int ORD = S2 ? 2 : 1;
   constexpr int H = ORD;
   constexpr int W = NC + 2 * H;
   constexpr int NW = S2 ? 4 : 3;               // planes in the column window
   constexpr bool SV = S2 && VISC;
-  __shared__ RF sp[NV][W][W];                  // primitives of the plane (cross region)
-  __shared__ RF sfx[NV][NC][NC + 1];           // x fluxes [v][j-1][s]
-  __shared__ RF sfy[NV][NC + 1][NC];           // y fluxes [v][s][i-1]
+  constexpr int NXF = (NC + 1) * NC;           // x (or y) interfaces per plane
+  __shared__ RF sp[NV][W * W];                 // primitives of the plane (cross region)
+  __shared__ RF sfx[NV][NC * (NC + 1)];        // x fluxes [v][(j-1)*17 + s]
+  __shared__ RF sfy[NV][(NC + 1) * NC];        // y fluxes [v][s*16 + (i-1)]
   __shared__ RF sun[SV ? 2 : 1][SV ? NV : 1][NC + 2][NC + 2];  // UP_new planes + half-step ghost ring
   __shared__ double sred[4];
 
@@ -205,12 +227,13 @@ __global__ void __launch_bounds__(256) k_stage(FusedArgs a) {
     RF fzp[NV];                                // z flux (k-1 | k)
     double unew_prev[MAXV], unew_prev2[MAXV];  // UP_new at planes k-1, k-2 (viscosity)
     // preload so that the shift at k = 0 produces the window of plane 0
+#pragma unroll
     for (int w = 1; w < NW; w++) {
-      int kk = w - 2;                          // S2: -1, 0, 1 ; S1: -1, 0
+      const int kk = w - 2;                    // S2: -1, 0, 1 ; S1: -1, 0
       if (!S2 && kk < 0) continue;
-      if (S2) load_cell<ST, NV, true>(a, UPin, b, i, j, kk, ucol);
+      if constexpr (S2) load_cell<ST, NV, true>(a, UPin, b, i, j, kk, ucol);
       else load_cell<double, NV, false>(a, a.U, b, i, j, kk, ucol);
-      if (S2 && kk == 0) {
+      if (SV && kk == 0) {
 #pragma unroll
         for (int v = 0; v < NV; v++) ughost_lo[v] = ucol[v];
       }
@@ -224,10 +247,10 @@ __global__ void __launch_bounds__(256) k_stage(FusedArgs a) {
 #pragma unroll
         for (int v = 0; v < NV; v++) win[w][v] = win[w + 1][v];
       {
-        int kn = S2 ? k + 2 : k + 1;
-        if (S2) load_cell<ST, NV, true>(a, UPin, b, i, j, kn, ucol);
+        const int kn = S2 ? k + 2 : k + 1;
+        if constexpr (S2) load_cell<ST, NV, true>(a, UPin, b, i, j, kn, ucol);
         else load_cell<double, NV, false>(a, a.U, b, i, j, kn, ucol);
-        if (S2 && kn == NC + 1) {
+        if (SV && kn == NC + 1) {
 #pragma unroll
           for (int v = 0; v < NV; v++) ughost_hi[v] = ucol[v];
         }
@@ -239,10 +262,20 @@ __global__ void __launch_bounds__(256) k_stage(FusedArgs a) {
         RF pll[NV], pl[NV], pr[NV], prr[NV];
 #pragma unroll
         for (int v = 0; v < NV; v++) {
-          if (S2) { pll[v] = win[0][v]; pl[v] = win[1][v]; pr[v] = win[2][v]; prr[v] = win[3][v]; }
-          else { pl[v] = win[1][v]; pr[v] = win[2][v]; }
+          pl[v] = win[1][v];
+          pr[v] = win[2][v];
+          if constexpr (S2) { pll[v] = win[0][v]; prr[v] = win[3][v]; }
         }
-        iface_flux<NV, ORD, RF>(P, 2, pll, pl, pr, prr, fzc);
+        rot<NV>(2, pl);
+        rot<NV>(2, pr);
+        if constexpr (S2) {
+          rot<NV>(2, pll);
+          rot<NV>(2, prr);
+#pragma unroll
+          for (int v = 0; v < NV; v++) limit1(pll[v], pl[v], pr[v], prr[v], P.limiter);
+        }
+        riemann_t<SOLVER>(P, pl, pr, fzc);
+        rot<NV>(2, fzc);
       }
       if (k == 0) {
 #pragma unroll
@@ -252,55 +285,61 @@ __global__ void __launch_bounds__(256) k_stage(FusedArgs a) {
       __syncthreads();   // previous plane's LDS readers are done
       // own primitives of plane k into LDS; halo threads add the face halo
 #pragma unroll
-      for (int v = 0; v < NV; v++) sp[v][j - 1 + H][i - 1 + H] = win[1][v];
+      for (int v = 0; v < NV; v++) sp[v][(j - 1 + H) * W + (i - 1 + H)] = win[1][v];
       if (tid < 4 * H * NC) {
-        int part = tid / (2 * H * NC);
-        int r = tid % (2 * H * NC);
-        int side = r / (H * NC);
-        int g = (r % (H * NC)) / NC + 1;
-        int aa = r % NC + 1;
+        const int part = tid / (2 * H * NC);
+        const int r = tid % (2 * H * NC);
+        const int side = r / (H * NC);
+        const int g = (r % (H * NC)) / NC + 1;
+        const int aa = r % NC + 1;
         int hi, hj;
         if (part == 0) { hi = side ? NC + g : 1 - g; hj = aa; }
         else { hj = side ? NC + g : 1 - g; hi = aa; }
         double uh[MAXV];
         RF ph[MAXV];
-        if (S2) load_cell<ST, NV, true>(a, UPin, b, hi, hj, k, uh);
+        if constexpr (S2) load_cell<ST, NV, true>(a, UPin, b, hi, hj, k, uh);
         else load_cell<double, NV, false>(a, a.U, b, hi, hj, k, uh);
         prims_r<NV, RF>(c, uh, ph, !S2);
 #pragma unroll
-        for (int v = 0; v < NV; v++) sp[v][hj - 1 + H][hi - 1 + H] = ph[v];
+        for (int v = 0; v < NV; v++) sp[v][(hj - 1 + H) * W + (hi - 1 + H)] = ph[v];
         if (SV && g == 1) {
 #pragma unroll
           for (int v = 0; v < NV; v++) sun[k & 1][SV ? v : 0][hj][hi] = (RF)uh[v];
         }
       }
       __syncthreads();
-      // x and y fluxes of plane k: 272 + 272 interfaces
-      for (int t = tid; t < 2 * (NC + 1) * NC; t += 256) {
-        int dim = t >= (NC + 1) * NC;
-        int r = dim ? t - (NC + 1) * NC : t;
-        int s = r % (NC + 1);     // interface (s | s+1)
-        int aa = r / (NC + 1) + 1;
+      // x and y fluxes of plane k: 272 + 272 interfaces over 256 threads
+      for (int t = tid; t < 2 * NXF; t += 256) {
+        const int d = t >= NXF;               // 0: x, 1: y
+        const int r = d ? t - NXF : t;
+        const int s = r % (NC + 1);           // interface (s | s+1) along d
+        const int aa = r / (NC + 1);          // row (0-based) across d
+        // LDS position of cell s-1 along d, and the stride along d
+        const int base = d ? (s - 2 + H) * W + (aa + H) : (aa + H) * W + (s - 2 + H);
+        const int st = d ? W : 1;
         RF pll[NV], pl[NV], pr[NV], prr[NV], ff[NV];
 #pragma unroll
         for (int v = 0; v < NV; v++) {
-          if (dim == 0) {
-            pl[v] = sp[v][aa - 1 + H][s - 1 + H];
-            pr[v] = sp[v][aa - 1 + H][s + H];
-            if (ORD == 2) { pll[v] = sp[v][aa - 1 + H][s - 2 + H]; prr[v] = sp[v][aa - 1 + H][s + 1 + H]; }
-          } else {
-            pl[v] = sp[v][s - 1 + H][aa - 1 + H];
-            pr[v] = sp[v][s + H][aa - 1 + H];
-            if (ORD == 2) { pll[v] = sp[v][s - 2 + H][aa - 1 + H]; prr[v] = sp[v][s + 1 + H][aa - 1 + H]; }
-          }
+          pl[v] = sp[v][base + st];
+          pr[v] = sp[v][base + 2 * st];
+          if constexpr (S2) { pll[v] = sp[v][base]; prr[v] = sp[v][base + 3 * st]; }
         }
-        iface_flux<NV, ORD, RF>(P, dim, pll, pl, pr, prr, ff);
-        if (dim == 0) {
+        rot<NV>(d, pl);
+        rot<NV>(d, pr);
+        if constexpr (S2) {
+          rot<NV>(d, pll);
+          rot<NV>(d, prr);
 #pragma unroll
-          for (int v = 0; v < NV; v++) sfx[v][aa - 1][s] = ff[v];
+          for (int v = 0; v < NV; v++) limit1(pll[v], pl[v], pr[v], prr[v], P.limiter);
+        }
+        riemann_t<SOLVER>(P, pl, pr, ff);
+        rot<NV>(d, ff);
+        if (d == 0) {
+#pragma unroll
+          for (int v = 0; v < NV; v++) sfx[v][aa * (NC + 1) + s] = ff[v];
         } else {
 #pragma unroll
-          for (int v = 0; v < NV; v++) sfy[v][s][aa - 1] = ff[v];
+          for (int v = 0; v < NV; v++) sfy[v][s * NC + aa] = ff[v];
         }
       }
       __syncthreads();
@@ -317,17 +356,17 @@ __global__ void __launch_bounds__(256) k_stage(FusedArgs a) {
 #pragma unroll
             for (int v = 0; v < NV; v++) ub[v] = w[v];
         }
+        // flux differences formed in FP64 (exact for FP32 operands), so the
+        // sum over cells telescopes
 #pragma unroll
         for (int v = 0; v < NV; v++)
-          // flux differences formed in FP64 (exact for FP32 operands), so
-          // the sum over cells telescopes
-          unew[v] = ub[v] + dtdx * ((double)sfx[v][j - 1][i - 1] - (double)sfx[v][j - 1][i]) +
-                    dtdy * ((double)sfy[v][j - 1][i - 1] - (double)sfy[v][j][i - 1]) +
+          unew[v] = ub[v] + dtdx * ((double)sfx[v][tj * (NC + 1) + ti] - (double)sfx[v][tj * (NC + 1) + ti + 1]) +
+                    dtdy * ((double)sfy[v][tj * NC + ti] - (double)sfy[v][(tj + 1) * NC + ti]) +
                     dtdz * ((double)fzp[v] - (double)fzc[v]);
         if (NV == 8 && c.eight_wave) {
           // Powell source (source/sources.f90:38-95) from the stage primitives
-          RF divB = (sp[5][j - 1 + H][i + H] - sp[5][j - 1 + H][i - 2 + H]) * idx2 +
-                    (sp[6][j + H][i - 1 + H] - sp[6][j - 2 + H][i - 1 + H]) * idy2 +
+          const int q = (j - 1 + H) * W + (i - 1 + H);
+          RF divB = (sp[5][q + 1] - sp[5][q - 1]) * idx2 + (sp[6][q + W] - sp[6][q - W]) * idy2 +
                     (win[2][7] - win[0][7]) * idz2;
           const RF *pp = win[1];
           unew[1] += dtp * (double)(-divB * pp[5]);
@@ -342,7 +381,7 @@ __global__ void __launch_bounds__(256) k_stage(FusedArgs a) {
 #pragma unroll
       for (int v = 0; v < NV; v++) fzp[v] = fzc[v];
 
-      if (!S2) {
+      if constexpr (!S2) {
         ST *UPo = UPout + (size_t)b * NV * NINT + lin(i, j, k);
 #pragma unroll
         for (int v = 0; v < NV; v++) UPo[v * NINT] = (ST)unew[v];
@@ -350,19 +389,18 @@ __global__ void __launch_bounds__(256) k_stage(FusedArgs a) {
       }
 
       // ---- stage 2 epilogue: viscosity (one plane behind), prims, cooling, dt
-      if (VISC) {
+      if constexpr (SV) {
 #pragma unroll
-        for (int v = 0; v < NV; v++) sun[k & 1][SV ? v : 0][j][i] = (RF)unew[v];
+        for (int v = 0; v < NV; v++) sun[k & 1][v][j][i] = (RF)unew[v];
         __syncthreads();
         if (k > 1) {
           const int pb = (k - 1) & 1;
           double ufin[MAXV];
 #pragma unroll
           for (int v = 0; v < NV; v++) {
-            const int vv = SV ? v : 0;
             double below = (k - 2 == 0) ? ughost_lo[v] : unew_prev2[v];
-            double lap = (double)sun[pb][vv][j][i + 1] + (double)sun[pb][vv][j][i - 1] +
-                         (double)sun[pb][vv][j + 1][i] + (double)sun[pb][vv][j - 1][i] + below + unew[v] -
+            double lap = (double)sun[pb][v][j][i + 1] + (double)sun[pb][v][j][i - 1] +
+                         (double)sun[pb][v][j + 1][i] + (double)sun[pb][v][j - 1][i] + below + unew[v] -
                          6.0 * unew_prev[v];
             ufin[v] = unew_prev[v] + c.visc_eta * lap;
           }
@@ -374,23 +412,22 @@ __global__ void __launch_bounds__(256) k_stage(FusedArgs a) {
         finish_cell<NV>(a, b, i, j, k, dt, unew, dxl, dyl, dzl, dtmin);
       }
     }
-    if (SV) {
+    if constexpr (SV) {
       // plane 16: its upper neighbour is the half-step ghost at plane 17
       __syncthreads();
       const int pb = NC & 1;
       double ufin[MAXV];
 #pragma unroll
       for (int v = 0; v < NV; v++) {
-        const int vv = SV ? v : 0;
-        double lap = (double)sun[pb][vv][j][i + 1] + (double)sun[pb][vv][j][i - 1] + (double)sun[pb][vv][j + 1][i] +
-                     (double)sun[pb][vv][j - 1][i] + unew_prev2[v] + ughost_hi[v] - 6.0 * unew_prev[v];
+        double lap = (double)sun[pb][v][j][i + 1] + (double)sun[pb][v][j][i - 1] + (double)sun[pb][v][j + 1][i] +
+                     (double)sun[pb][v][j - 1][i] + unew_prev2[v] + ughost_hi[v] - 6.0 * unew_prev[v];
         ufin[v] = unew_prev[v] + c.visc_eta * lap;
       }
       finish_cell<NV>(a, b, i, j, NC, dt, ufin, dxl, dyl, dzl, dtmin);
     }
     __syncthreads();
   }
-  if (S2) {
+  if constexpr (S2) {
     // exact min: wavefront shuffles, then one 64-bit atomic per workgroup on
     // the bit pattern (positive doubles order like their bits)
     for (int off = 32; off > 0; off >>= 1) dtmin = fmin(dtmin, __shfl_xor(dtmin, off));
@@ -492,6 +529,32 @@ static unsigned long long morton(const BlockInfo &B, int maxlev) {
   return key;
 }
 
+// Block processing order: Morton order of the blocks' positions; with the
+// XCD map, workgroup w runs on XCD w % 8, so each XCD gets a contiguous
+// Morton range (neighbour halos then hit that XCD's L2).
+static int build_order(FusedState &f) {
+  const int nb = f.nblk;
+  std::vector<std::pair<unsigned long long, int>> keys(nb);
+  for (int b = 0; b < nb; b++) keys[b] = {morton(f.host_info[b], f.maxlev), b};
+  std::sort(keys.begin(), keys.end());
+  const int nx = 8;
+  std::vector<int> ord(nb, -1);
+  if (f.xcd_map && nb >= 64) {
+    int full = (nb / nx) * nx;
+    int perx = full / nx;
+    for (int r = 0; r < full; r++) {
+      int xcd = r / perx, within = r % perx;
+      ord[within * nx + xcd] = keys[r].second;
+    }
+    for (int r = full; r < nb; r++) ord[r] = keys[r].second;
+  } else {
+    for (int r = 0; r < nb; r++) ord[r] = keys[r].second;
+  }
+  f.host_order = ord;
+  FCHK(hipMemcpy(f.order, ord.data(), nb * sizeof(int), hipMemcpyHostToDevice));
+  return 0;
+}
+
 int fused_alloc(FusedState &f, const DevCfg &c, const std::vector<BlockInfo> &info, hipStream_t st) {
   fused_free(f);
   f.nblk = c.nblk;
@@ -501,26 +564,9 @@ int fused_alloc(FusedState &f, const DevCfg &c, const std::vector<BlockInfo> &in
   FCHK(hipMalloc(&f.UP, n * (f.fp64 ? sizeof(double) : sizeof(float))));
   FCHK(hipMalloc(&f.order, c.nblk * sizeof(int)));
   FCHK(hipMalloc(&f.step, sizeof(StepState)));
-  // processing order: Morton order; with the XCD map, workgroup w runs on
-  // XCD w % 8, so consecutive Morton blocks go to workgroups 8 apart
-  std::vector<std::pair<unsigned long long, int>> keys(c.nblk);
-  for (int b = 0; b < c.nblk; b++) keys[b] = {morton(info[b], c.maxlev), b};
-  std::sort(keys.begin(), keys.end());
-  const int nx = 8;
-  std::vector<int> ord(c.nblk, -1);
-  if (f.xcd_map && c.nblk >= 64) {
-    int full = (c.nblk / nx) * nx;
-    int perx = full / nx;
-    for (int r = 0; r < full; r++) {
-      int xcd = r / perx, within = r % perx;
-      ord[within * nx + xcd] = keys[r].second;
-    }
-    for (int r = full; r < c.nblk; r++) ord[r] = keys[r].second;
-  } else {
-    for (int r = 0; r < c.nblk; r++) ord[r] = keys[r].second;
-  }
-  f.host_order = ord;
-  FCHK(hipMemcpy(f.order, ord.data(), c.nblk * sizeof(int), hipMemcpyHostToDevice));
+  f.host_info = info;
+  f.maxlev = c.maxlev;
+  if (build_order(f)) return 1;
   for (int e = 0; e < 4; e++) FCHK(hipEventCreate(&f.ev[e]));
   (void)st;
   return 0;
@@ -596,27 +642,33 @@ int fused_timestep(FusedState &f, const DevCfg &c, hipStream_t st, double *dt_ra
   return 0;
 }
 
+template <typename RF, typename ST, int NV, int SOLVER>
+static void launch_stage_s(const FusedArgs &a, int grid, bool visc, int stage, hipStream_t st) {
+  if (stage == 1)
+    hipLaunchKernelGGL((k_stage<NV, false, false, RF, ST, SOLVER>), dim3(grid), dim3(256), 0, st, a);
+  else if (visc)
+    hipLaunchKernelGGL((k_stage<NV, true, true, RF, ST, SOLVER>), dim3(grid), dim3(256), 0, st, a);
+  else
+    hipLaunchKernelGGL((k_stage<NV, true, false, RF, ST, SOLVER>), dim3(grid), dim3(256), 0, st, a);
+}
+
 template <typename RF, typename ST>
-static void launch_stage_t(const FusedArgs &a, int grid, int nv, bool visc, int stage, hipStream_t st) {
-#define WX_LAUNCH(NVv, S2v, Vv) hipLaunchKernelGGL((k_stage<NVv, S2v, Vv, RF, ST>), dim3(grid), dim3(256), 0, st, a)
+static void launch_stage_t(const FusedArgs &a, int grid, int nv, int solver, bool visc, int stage, hipStream_t st) {
   if (nv == 8) {
-    if (stage == 1) WX_LAUNCH(8, false, false);
-    else if (visc) WX_LAUNCH(8, true, true);
-    else WX_LAUNCH(8, true, false);
+    if (solver == SOLVER_HLLD) launch_stage_s<RF, ST, 8, SOLVER_HLLD>(a, grid, visc, stage, st);
+    else launch_stage_s<RF, ST, 8, SOLVER_HLLE>(a, grid, visc, stage, st);
   } else {
-    if (stage == 1) WX_LAUNCH(5, false, false);
-    else if (visc) WX_LAUNCH(5, true, true);
-    else WX_LAUNCH(5, true, false);
+    if (solver == SOLVER_HLLC) launch_stage_s<RF, ST, 5, SOLVER_HLLC>(a, grid, visc, stage, st);
+    else launch_stage_s<RF, ST, 5, SOLVER_HLL>(a, grid, visc, stage, st);
   }
-#undef WX_LAUNCH
 }
 
 static void launch_stage(FusedState &f, const DevCfg &c, hipStream_t st, int stage) {
   FusedArgs a = make_args(f, c);
   int grid = f.grid > 0 ? std::min(f.grid, f.nblk) : f.nblk;
   bool visc = c.visc_eta != 0.0;
-  if (f.fp64) launch_stage_t<double, double>(a, grid, c.nv, visc, stage, st);
-  else launch_stage_t<float, float>(a, grid, c.nv, visc, stage, st);
+  if (f.fp64) launch_stage_t<double, double>(a, grid, c.nv, c.pd.solver, visc, stage, st);
+  else launch_stage_t<float, float>(a, grid, c.nv, c.pd.solver, visc, stage, st);
 }
 
 static void enqueue_step(FusedState &f, const DevCfg &c, hipStream_t st) {
@@ -686,9 +738,24 @@ int fused_time_stages(FusedState &f, const DevCfg &c, hipStream_t st, int reps) 
   return 0;
 }
 
+// kind 0: stage-kernel grid (0 = one workgroup per block, else a persistent
+//         grid of that many workgroups striding over the block list)
+// kind 1: XCD-aware block order (1) or plain Morton order (0)
 int fused_set_tuning(FusedState &f, int kind, int value) {
-  if (kind == 0) { f.grid = value; if (f.exec) { hipGraphExecDestroy(f.exec); hipGraphDestroy(f.graph); f.exec = nullptr; f.graph = nullptr; } return 0; }
-  return 1;
+  if (kind != 0 && kind != 1) return 1;
+  if (f.exec) {
+    hipGraphExecDestroy(f.exec);
+    hipGraphDestroy(f.graph);
+    f.exec = nullptr;
+    f.graph = nullptr;
+  }
+  if (kind == 0) {
+    if (value < 0) return 1;
+    f.grid = value;
+    return 0;
+  }
+  f.xcd_map = value != 0;
+  return f.nblk > 0 ? build_order(f) : 0;
 }
 
 void fused_last_timing(const FusedState &f, double *s1, double *s2, double *tot) {
