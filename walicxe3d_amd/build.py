@@ -29,7 +29,9 @@ UNITS = {
     "wx_api.hip": ["-ffp-contract=off"],
     # no contraction either: a face flux computed by the two blocks that share
     # it must round identically for the update to stay conservative
-    "wx_fused.hip": ["-ffp-contract=off"],
+    # FP32 division and square root with the hardware's approximate lowering
+    # (the FP64 arithmetic of the file is unaffected)
+    "wx_fused.hip": ["-ffp-contract=off", "-fno-hip-fp32-correctly-rounded-divide-sqrt"],
 }
 
 

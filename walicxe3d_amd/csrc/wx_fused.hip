@@ -165,18 +165,47 @@ __device__ __forceinline__ double dt_cell(const DevCfg &c, const double *p, doub
   return r;
 }
 
-// stage-2 epilogue of one cell: primitives (FP64), cooling, write, dt
+// CFL candidate of one cell from FP32 primitives
 template <int NV>
+__device__ __forceinline__ float dt_cell_f(const DevCfg &c, const float *p, float dx, float dy, float dz) {
+  float r;
+  if (NV == 8 && c.pf.mhd) {
+    float cfx = cfast1(c.pf, p[4], p[0], p[5], p[6], p[7], p[5]);
+    float cfy = cfast1(c.pf, p[4], p[0], p[5], p[6], p[7], p[6]);
+    float cfz = cfast1(c.pf, p[4], p[0], p[5], p[6], p[7], p[7]);
+    r = fminf(fminf(dx / (fabsf(p[1]) + cfx), dy / (fabsf(p[2]) + cfy)), dz / (fabsf(p[3]) + cfz));
+  } else {
+    float cs = sound(c.pf, p);
+    r = fminf(fminf(dx / (fabsf(p[1]) + cs), dy / (fabsf(p[2]) + cs)), dz / (fabsf(p[3]) + cs));
+  }
+  return r;
+}
+
+// stage-2 epilogue of one cell: write U^{n+1}, primitives, cooling, dt
+// candidate.  With cooling (or in the FP64 variant) the primitives and the
+// CFL candidate are FP64; otherwise the candidate comes from FP32
+// primitives (the min itself stays exact).
+template <int NV, typename RF>
 __device__ __forceinline__ void finish_cell(const FusedArgs &a, int b, int i, int j, int k, double dt, double *u,
                                             double dx, double dy, double dz, double &dtmin) {
   const DevCfg &c = a.c;
+  double *Uo = a.Uout + (size_t)b * NV * NINT + lin(i, j, k);
+  if (sizeof(RF) == 4 && !c.cooling) {
+#pragma unroll
+    for (int v = 0; v < NV; v++) Uo[v * NINT] = u[v];
+    float uf[MAXV], pf[MAXV];
+#pragma unroll
+    for (int v = 0; v < NV; v++) uf[v] = (float)u[v];
+    flow2prim(c.pf, uf, pf);
+    dtmin = fmin(dtmin, (double)dt_cell_f<NV>(c, pf, (float)dx, (float)dy, (float)dz));
+    return;
+  }
   double pd[MAXV];
   flow2prim(c.pd, u, pd);
   if (c.cooling) {
     double e;
     if (cool_cell(c, dt, pd, &e)) u[4] = e;
   }
-  double *Uo = a.Uout + (size_t)b * NV * NINT + lin(i, j, k);
 #pragma unroll
   for (int v = 0; v < NV; v++) Uo[v * NINT] = u[v];
   dtmin = fmin(dtmin, dt_cell<NV>(c, pd, dx, dy, dz));
@@ -192,10 +221,11 @@ __global__ void __launch_bounds__(256, 2) k_stage(FusedArgs a) {
   constexpr int ORD = S2 ? 2 : 1;
   constexpr int H = ORD;
   constexpr int W = NC + 2 * H;
+  constexpr int WP = W | 1;                    // odd LDS row pitch (bank spread of y-direction reads)
   constexpr int NW = S2 ? 4 : 3;               // planes in the column window
   constexpr bool SV = S2 && VISC;
   constexpr int NXF = (NC + 1) * NC;           // x (or y) interfaces per plane
-  __shared__ RF sp[NV][W * W];                 // primitives of the plane (cross region)
+  __shared__ RF sp[NV][W * WP];                 // primitives of the plane (cross region)
   __shared__ RF sfx[NV][NC * (NC + 1)];        // x fluxes [v][(j-1)*17 + s]
   __shared__ RF sfy[NV][(NC + 1) * NC];        // y fluxes [v][s*16 + (i-1)]
   __shared__ RF sun[SV ? 2 : 1][SV ? NV : 1][NC + 2][NC + 2];  // UP_new planes + half-step ghost ring
@@ -285,7 +315,7 @@ __global__ void __launch_bounds__(256, 2) k_stage(FusedArgs a) {
       __syncthreads();   // previous plane's LDS readers are done
       // own primitives of plane k into LDS; halo threads add the face halo
 #pragma unroll
-      for (int v = 0; v < NV; v++) sp[v][(j - 1 + H) * W + (i - 1 + H)] = win[1][v];
+      for (int v = 0; v < NV; v++) sp[v][(j - 1 + H) * WP + (i - 1 + H)] = win[1][v];
       if (tid < 4 * H * NC) {
         const int part = tid / (2 * H * NC);
         const int r = tid % (2 * H * NC);
@@ -301,7 +331,7 @@ __global__ void __launch_bounds__(256, 2) k_stage(FusedArgs a) {
         else load_cell<double, NV, false>(a, a.U, b, hi, hj, k, uh);
         prims_r<NV, RF>(c, uh, ph, !S2);
 #pragma unroll
-        for (int v = 0; v < NV; v++) sp[v][(hj - 1 + H) * W + (hi - 1 + H)] = ph[v];
+        for (int v = 0; v < NV; v++) sp[v][(hj - 1 + H) * WP + (hi - 1 + H)] = ph[v];
         if (SV && g == 1) {
 #pragma unroll
           for (int v = 0; v < NV; v++) sun[k & 1][SV ? v : 0][hj][hi] = (RF)uh[v];
@@ -315,8 +345,8 @@ __global__ void __launch_bounds__(256, 2) k_stage(FusedArgs a) {
         const int s = r % (NC + 1);           // interface (s | s+1) along d
         const int aa = r / (NC + 1);          // row (0-based) across d
         // LDS position of cell s-1 along d, and the stride along d
-        const int base = d ? (s - 2 + H) * W + (aa + H) : (aa + H) * W + (s - 2 + H);
-        const int st = d ? W : 1;
+        const int base = d ? (s - 2 + H) * WP + (aa + H) : (aa + H) * WP + (s - 2 + H);
+        const int st = d ? WP : 1;
         RF pll[NV], pl[NV], pr[NV], prr[NV], ff[NV];
 #pragma unroll
         for (int v = 0; v < NV; v++) {
@@ -365,8 +395,8 @@ __global__ void __launch_bounds__(256, 2) k_stage(FusedArgs a) {
                     dtdz * ((double)fzp[v] - (double)fzc[v]);
         if (NV == 8 && c.eight_wave) {
           // Powell source (source/sources.f90:38-95) from the stage primitives
-          const int q = (j - 1 + H) * W + (i - 1 + H);
-          RF divB = (sp[5][q + 1] - sp[5][q - 1]) * idx2 + (sp[6][q + W] - sp[6][q - W]) * idy2 +
+          const int q = (j - 1 + H) * WP + (i - 1 + H);
+          RF divB = (sp[5][q + 1] - sp[5][q - 1]) * idx2 + (sp[6][q + WP] - sp[6][q - WP]) * idy2 +
                     (win[2][7] - win[0][7]) * idz2;
           const RF *pp = win[1];
           unew[1] += dtp * (double)(-divB * pp[5]);
@@ -404,12 +434,12 @@ __global__ void __launch_bounds__(256, 2) k_stage(FusedArgs a) {
                          6.0 * unew_prev[v];
             ufin[v] = unew_prev[v] + c.visc_eta * lap;
           }
-          finish_cell<NV>(a, b, i, j, k - 1, dt, ufin, dxl, dyl, dzl, dtmin);
+          finish_cell<NV, RF>(a, b, i, j, k - 1, dt, ufin, dxl, dyl, dzl, dtmin);
         }
 #pragma unroll
         for (int v = 0; v < NV; v++) { unew_prev2[v] = unew_prev[v]; unew_prev[v] = unew[v]; }
       } else {
-        finish_cell<NV>(a, b, i, j, k, dt, unew, dxl, dyl, dzl, dtmin);
+        finish_cell<NV, RF>(a, b, i, j, k, dt, unew, dxl, dyl, dzl, dtmin);
       }
     }
     if constexpr (SV) {
@@ -423,7 +453,7 @@ __global__ void __launch_bounds__(256, 2) k_stage(FusedArgs a) {
                      (double)sun[pb][v][j - 1][i] + unew_prev2[v] + ughost_hi[v] - 6.0 * unew_prev[v];
         ufin[v] = unew_prev[v] + c.visc_eta * lap;
       }
-      finish_cell<NV>(a, b, i, j, NC, dt, ufin, dxl, dyl, dzl, dtmin);
+      finish_cell<NV, RF>(a, b, i, j, NC, dt, ufin, dxl, dyl, dzl, dtmin);
     }
     __syncthreads();
   }

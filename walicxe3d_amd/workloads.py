@@ -270,6 +270,11 @@ _reg(_w(name="sedov", mhd=False, solver="hllc", limiter="minmod", gamma_expr="5.
         gamma=5.0 / 3.0, cfl=0.4, visc_eta=5.0e-3, eight_wave=False, bc=FREE6,
         nroot=(1, 1, 1), maxlev=3, base_level=3, phys=(1.0, 1.0, 1.0), ic="sedov", seed=1, nsteps=100,
         description="3D Sedov-type blast, 64^3 cells as 4x4x4 blocks of 16^3, HLLC, minmod"))
+# C1 at 256^3 (GPU throughput of the hydro path at the Orszag-Tang size)
+_reg(_w(name="sedov256", mhd=False, solver="hllc", limiter="minmod", gamma_expr="5.0/3.0",
+        gamma=5.0 / 3.0, cfl=0.4, visc_eta=5.0e-3, eight_wave=False, bc=FREE6,
+        nroot=(1, 1, 1), maxlev=5, base_level=5, phys=(1.0, 1.0, 1.0), ic="sedov", seed=1, nsteps=20,
+        oracle=False, description="3D Sedov-type blast, 256^3 cells (4096 blocks of 16^3), HLLC, minmod"))
 # C2: Brio-Wu, 32x1x1 blocks (512x16x16 cells), HLLD + minmod, to t=0.1
 _reg(_w(name="briowu", mhd=True, solver="hlld", limiter="minmod", gamma_expr="2.0",
         gamma=2.0, cfl=0.4, visc_eta=0.0, eight_wave=True, bc=FREE6,
