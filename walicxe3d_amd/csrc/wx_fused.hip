@@ -49,8 +49,13 @@ template <> __device__ __forceinline__ const Phys<float> &phys<float>(const DevC
 template <> __device__ __forceinline__ const Phys<double> &phys<double>(const DevCfg &c) { return c.pd; }
 
 // Riemann solver chosen at compile time (one solver's code per kernel)
+#ifdef WX_NOINLINE_RIEMANN
+#define WX_RIEMANN_INLINE __noinline__
+#else
+#define WX_RIEMANN_INLINE __forceinline__
+#endif
 template <int SOLVER, typename RF>
-__device__ __forceinline__ void riemann_t(const Phys<RF> &P, const RF *pl, const RF *pr, RF *ff) {
+__device__ WX_RIEMANN_INLINE void riemann_t(const Phys<RF> &P, const RF *pl, const RF *pr, RF *ff) {
   if constexpr (SOLVER == SOLVER_HLLC) primfhllc(P, pl, pr, ff);
   else if constexpr (SOLVER == SOLVER_HLLD) primfhlld(P, pl, pr, ff);
   else if constexpr (SOLVER == SOLVER_HLLE) primfhlle(P, pl, pr, ff);
@@ -214,10 +219,13 @@ __device__ __forceinline__ void finish_cell(const FusedArgs &a, int b, int i, in
 // ---------------------------------------------------------------------------
 // Stage kernel.  S2 = second stage; VISC = artificial viscosity active;
 // RF = flux/reconstruction arithmetic; ST = storage type of UP; SOLVER =
-// Riemann solver.  Launch bound: 256 threads, at least two waves per SIMD.
+// Riemann solver.  Launch bound: 256 threads, at least three waves per SIMD.
 // ---------------------------------------------------------------------------
+#ifndef WX_STAGE_MIN_WAVES
+#define WX_STAGE_MIN_WAVES 3
+#endif
 template <int NV, bool S2, bool VISC, typename RF, typename ST, int SOLVER>
-__global__ void __launch_bounds__(256, 2) k_stage(FusedArgs a) {
+__global__ void __launch_bounds__(256, WX_STAGE_MIN_WAVES) k_stage(FusedArgs a) {
   constexpr int ORD = S2 ? 2 : 1;
   constexpr int H = ORD;
   constexpr int W = NC + 2 * H;

@@ -16,7 +16,7 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIBPATH = os.path.join(HERE, "libwalicxe_hip.so")
+LIBPATH = os.environ.get("WX_LIB") or os.path.join(HERE, "libwalicxe_hip.so")
 
 PLAIN_FP64 = 0
 FUSED_FP32 = 1
