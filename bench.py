@@ -162,6 +162,7 @@ def main():
         rel = np.where(s > 0, d / np.maximum(s, 1e-300), d)
         names = ["rho", "mx", "my", "mz", "E", "bx", "by", "bz"][: w.nv]
         err = {"steps": args.err_steps, "reference": "plain FP64 GPU form (bit-exact with the FP64 oracle)",
+               "rel_l1_all_conserved": float("%.3e" % (d.sum() / s.sum())),
                "rel_l1": {n: float("%.3e" % v) for n, v in zip(names, rel)}, "max": float("%.3e" % rel.max())}
         log("fused rel L1 vs FP64 after %d steps: %s" % (args.err_steps, err["rel_l1"]))
 

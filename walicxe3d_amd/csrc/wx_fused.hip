@@ -57,7 +57,13 @@ template <> __device__ __forceinline__ const Phys<double> &phys<double>(const De
 template <int SOLVER, typename RF>
 __device__ WX_RIEMANN_INLINE void riemann_t(const Phys<RF> &P, const RF *pl, const RF *pr, RF *ff) {
   if constexpr (SOLVER == SOLVER_HLLC) primfhllc(P, pl, pr, ff);
-  else if constexpr (SOLVER == SOLVER_HLLD) primfhlld(P, pl, pr, ff);
+  else if constexpr (SOLVER == SOLVER_HLLD) {
+#ifdef WX_HLLD_BRANCHES
+    primfhlld(P, pl, pr, ff);
+#else
+    primfhlld_sel(P, pl, pr, ff);
+#endif
+  }
   else if constexpr (SOLVER == SOLVER_HLLE) primfhlle(P, pl, pr, ff);
   else primfhll(P, pl, pr, ff);
 }

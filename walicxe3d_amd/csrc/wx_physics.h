@@ -354,6 +354,94 @@ __host__ __device__ inline void primfhlld(const Phys<R> &c, const R *pl, const R
   for (int q = 0; q < 8; q++) ff[q] = R(NAN);
 }
 
+// primfhlld evaluated with one code path for the four inner regions (UL*,
+// UR*, UL**, UR**): both star states are formed once, the region's values
+// are picked with selects, and only the double-star states sit behind a
+// branch.  Every value is the expression primfhlld uses for its region, so
+// the two functions agree bit for bit; a wavefront whose lanes fall into
+// different regions no longer runs each region's code in turn.
+template <typename R>
+__host__ __device__ inline void primfhlld_sel(const Phys<R> &c, const R *pl, const R *pr, R *ff) {
+  const R cv = c.cv;
+  R csl = cfastX(c, pl), csr = cfastX(c, pr);
+  R sr = rfmax(pl[1] + csl, pr[1] + csr);
+  R sl = rfmin(pl[1] - csl, pr[1] - csr);
+  if (sl > R(0)) { xflux(c, pl, ff); return; }
+  if (sr < R(0)) { xflux(c, pr, ff); return; }
+  R bx = R(0.5) * (pl[5] + pr[5]);
+  R signBx = rcopysign(R(1), bx);
+  R pTL = pl[4] + R(0.5) * (bx * bx + pl[6] * pl[6] + pl[7] * pl[7]);
+  R pTR = pr[4] + R(0.5) * (bx * bx + pr[6] * pr[6] + pr[7] * pr[7]);
+  R slmul = sl - pl[1];
+  R srmur = sr - pr[1];
+  R rholul = pl[0] * pl[1];
+  R rhorur = pr[0] * pr[1];
+  R sM = (srmur * rhorur - slmul * rholul - pTR + pTL) / (srmur * pr[0] - slmul * pl[0]);
+  R srmsM = sr - sM;
+  R slmsM = sl - sM;
+  R rhostl = pl[0] * slmul / slmsM;
+  R rhostr = pr[0] * srmur / srmsM;
+  R sstl = sM - rabs(bx) / rsqrt_(rhostl);
+  R sstr = sM + rabs(bx) / rsqrt_(rhostr);
+  R pst = (srmur * pr[0] * pTL - slmul * pl[0] * pTR + pl[0] * pr[0] * srmur * slmul * (pr[1] - pl[1])) /
+          (srmur * pr[0] - slmul * pl[0]);
+  // region, in the order the reference tests it
+  const bool s1L = sstl >= R(0);
+  const bool s1R = !s1L && sstr <= R(0);
+  const bool d2L = !s1L && !s1R && sM >= R(0);
+  const bool d2R = !s1L && !s1R && !d2L && sM <= R(0);
+  if (!(s1L || s1R || d2L || d2R)) {
+    for (int q = 0; q < 8; q++) ff[q] = R(NAN);
+    return;
+  }
+  const bool isL = s1L || d2L;
+  const bool single = s1L || s1R;
+  R vstl, wstl, bystl, bzstl, vstr, wstr, bystr, bzstr;
+  hlld_star(pl, bx, slmul, slmsM, sM - pl[1], vstl, wstl, bystl, bzstl);
+  hlld_star(pr, bx, srmur, srmsM, sM - pr[1], vstr, wstr, bystr, bzstr);
+  // the side K whose star state the flux is built on
+  R pK[8];
+  for (int q = 0; q < 8; q++) pK[q] = isL ? pl[q] : pr[q];
+  const R sKmuK = isL ? slmul : srmur;
+  const R sKmsM = isL ? slmsM : srmsM;
+  const R rhostK = isL ? rhostl : rhostr;
+  const R pTK = isL ? pTL : pTR;
+  const R vstK = isL ? vstl : vstr, wstK = isL ? wstl : wstr;
+  const R bystK = isL ? bystl : bystr, bzstK = isL ? bzstl : bzstr;
+  R eK = R(0.5) * pK[0] * (pK[1] * pK[1] + pK[2] * pK[2] + pK[3] * pK[3]) + cv * pK[4] +
+         R(0.5) * (bx * bx + pK[6] * pK[6] + pK[7] * pK[7]);
+  R vdotbK = pK[1] * bx + pK[2] * pK[6] + pK[3] * pK[7];
+  R vstdotbstK = sM * bx + vstK * bystK + wstK * bzstK;
+  R estK = (sKmuK * eK - pTK * pK[1] + pst * sM + bx * (vdotbK - vstdotbstK)) / sKmsM;
+  R vq = vstK, wq = wstK, byq = bystK, bzq = bzstK, eq = estK, vdbq = vstdotbstK;
+  if (!single) {
+    R sql = rsqrt_(rhostl), sqr = rsqrt_(rhostr);
+    R dd = sql + sqr;
+    R vstst = (sql * vstl + sqr * vstr + (bystr - bystl) * signBx) / dd;
+    R wstst = (sql * wstl + sqr * wstr + (bzstr - bzstl) * signBx) / dd;
+    R sqlr = rsqrt_(rhostl * rhostr);
+    R bystst = (sql * bystr + sqr * bystl + sqlr * (vstr - vstl) * signBx) / dd;
+    R bzstst = (sql * bzstr + sqr * bzstl + sqlr * (wstr - wstl) * signBx) / dd;
+    R vststdotbstst = sM * bx + vstst * bystst + wstst * bzstst;
+    R eststl = estK - sql * (vstdotbstK - vststdotbstst) * signBx;
+    R eststr = estK + sqr * (vstdotbstK - vststdotbstst) * signBx;
+    vq = vstst;
+    wq = wstst;
+    byq = bystst;
+    bzq = bzstst;
+    eq = isL ? eststl : eststr;
+    vdbq = vststdotbstst;
+  }
+  ff[0] = rhostK * sM;
+  ff[1] = rhostK * (sM * sM) + pst - bx * bx;
+  ff[2] = rhostK * sM * vq - bx * byq;
+  ff[3] = rhostK * sM * wq - bx * bzq;
+  ff[4] = sM * (eq + pst) - bx * (vdbq);
+  ff[5] = R(0);
+  ff[6] = byq * sM - bx * vq;
+  ff[7] = bzq * sM - bx * wq;
+}
+
 // solver dispatch on x-aligned states
 template <typename R>
 __host__ __device__ inline void riemann(const Phys<R> &c, const R *pl, const R *pr, R *ff) {
