@@ -225,12 +225,14 @@ __device__ __forceinline__ void finish_cell(const FusedArgs &a, int b, int i, in
 // ---------------------------------------------------------------------------
 // Stage kernel.  S2 = second stage; VISC = artificial viscosity active;
 // RF = flux/reconstruction arithmetic; ST = storage type of UP; SOLVER =
-// Riemann solver.  Launch bound: 256 threads, at least three waves per SIMD.
+// Riemann solver; LIM = limiter fixed at compile time, or LIM_RUNTIME.
+// Launch bound: 256 threads, at least three waves per SIMD.
 // ---------------------------------------------------------------------------
+constexpr int LIM_RUNTIME = -100;
 #ifndef WX_STAGE_MIN_WAVES
 #define WX_STAGE_MIN_WAVES 3
 #endif
-template <int NV, bool S2, bool VISC, typename RF, typename ST, int SOLVER>
+template <int NV, bool S2, bool VISC, typename RF, typename ST, int SOLVER, int LIM>
 __global__ void __launch_bounds__(256, WX_STAGE_MIN_WAVES) k_stage(FusedArgs a) {
   constexpr int ORD = S2 ? 2 : 1;
   constexpr int H = ORD;
@@ -316,7 +318,7 @@ __global__ void __launch_bounds__(256, WX_STAGE_MIN_WAVES) k_stage(FusedArgs a) 
           rot<NV>(2, pll);
           rot<NV>(2, prr);
 #pragma unroll
-          for (int v = 0; v < NV; v++) limit1(pll[v], pl[v], pr[v], prr[v], P.limiter);
+          for (int v = 0; v < NV; v++) limit1(pll[v], pl[v], pr[v], prr[v], LIM == LIM_RUNTIME ? P.limiter : LIM);
         }
         riemann_t<SOLVER>(P, pl, pr, fzc);
         rot<NV>(2, fzc);
@@ -374,7 +376,7 @@ __global__ void __launch_bounds__(256, WX_STAGE_MIN_WAVES) k_stage(FusedArgs a) 
           rot<NV>(d, pll);
           rot<NV>(d, prr);
 #pragma unroll
-          for (int v = 0; v < NV; v++) limit1(pll[v], pl[v], pr[v], prr[v], P.limiter);
+          for (int v = 0; v < NV; v++) limit1(pll[v], pl[v], pr[v], prr[v], LIM == LIM_RUNTIME ? P.limiter : LIM);
         }
         riemann_t<SOLVER>(P, pl, pr, ff);
         rot<NV>(d, ff);
@@ -686,14 +688,28 @@ int fused_timestep(FusedState &f, const DevCfg &c, hipStream_t st, double *dt_ra
   return 0;
 }
 
+// second-stage kernels with the limiter fixed at compile time (minmod, in
+// the FP32 MHD form) or read at run time
+template <typename RF, typename ST, int NV, int SOLVER, bool VISC>
+static void launch_stage2(const FusedArgs &a, int grid, hipStream_t st) {
+  constexpr int MM = 1;  // LIMITER_MINMOD (source/constants.f90)
+  if constexpr (sizeof(RF) == 4 && NV == 8) {
+    if (a.c.pf.limiter == MM) {
+      hipLaunchKernelGGL((k_stage<NV, true, VISC, RF, ST, SOLVER, MM>), dim3(grid), dim3(256), 0, st, a);
+      return;
+    }
+  }
+  hipLaunchKernelGGL((k_stage<NV, true, VISC, RF, ST, SOLVER, LIM_RUNTIME>), dim3(grid), dim3(256), 0, st, a);
+}
+
 template <typename RF, typename ST, int NV, int SOLVER>
 static void launch_stage_s(const FusedArgs &a, int grid, bool visc, int stage, hipStream_t st) {
   if (stage == 1)
-    hipLaunchKernelGGL((k_stage<NV, false, false, RF, ST, SOLVER>), dim3(grid), dim3(256), 0, st, a);
+    hipLaunchKernelGGL((k_stage<NV, false, false, RF, ST, SOLVER, LIM_RUNTIME>), dim3(grid), dim3(256), 0, st, a);
   else if (visc)
-    hipLaunchKernelGGL((k_stage<NV, true, true, RF, ST, SOLVER>), dim3(grid), dim3(256), 0, st, a);
+    launch_stage2<RF, ST, NV, SOLVER, true>(a, grid, st);
   else
-    hipLaunchKernelGGL((k_stage<NV, true, false, RF, ST, SOLVER>), dim3(grid), dim3(256), 0, st, a);
+    launch_stage2<RF, ST, NV, SOLVER, false>(a, grid, st);
 }
 
 template <typename RF, typename ST>

@@ -1,0 +1,148 @@
+"""Profiling driver for rocprofv3 counter runs of the fused stage kernels.
+
+Run mode (the program to put after `rocprofv3 ... --`):
+    python3 -m walicxe3d_amd.profile_run run --workload ot256 --reps 2
+uploads the workload, advances one step through the step graph, then
+launches each stage kernel on its own `reps` times (so counters are
+collected per dispatch) and finally copies a 1 GiB device buffer (a plain
+copy kernel of known size, used to scale the memory counters).
+
+Summary mode:
+    python3 -m walicxe3d_amd.profile_run summarize --fetch DIR --write DIR \
+        --workload ot256 --out profiles/stage2_ot256_traffic.txt [--cmd "..."]
+reads the two counter_collection.csv files (FETCH_SIZE and WRITE_SIZE, KiB
+per dispatch), takes the copy kernel's readings as the scale (bytes moved /
+counter), and writes per-launch HBM traffic of the stage kernels next to
+their algorithmic bytes.
+"""
+from __future__ import annotations
+
+import argparse
+import collections
+import csv
+import glob
+import os
+import sys
+
+COPY_BYTES = 1 << 30
+
+
+def _run(args):
+    import numpy as np
+    import torch
+
+    from walicxe3d_amd import hip
+    from walicxe3d_amd import workloads as W
+
+    w = W.get(args.workload)
+    slots, blocks, U0 = W.initial_state(w)
+    g = hip.Walicxe3D(w, form=hip.FUSED_FP32, bids=np.asarray(blocks, dtype=np.int32))
+    g.upload(U0)
+    g.run(1)
+    g.time_stages(args.reps)
+    g.close()
+    x = torch.ones(COPY_BYTES // 4, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    y.copy_(x)
+    torch.cuda.synchronize()
+    print("profile_run done: %s, %d stage launches each, %d-byte copy" % (w.name, args.reps, COPY_BYTES),
+          flush=True)
+    return 0
+
+
+def _read(d):
+    paths = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
+    if not paths:
+        raise FileNotFoundError("no counter_collection.csv under %s" % d)
+    rows = []
+    for p in paths:
+        rows += list(csv.DictReader(open(p)))
+    return rows
+
+
+def _per_kernel(rows, counter):
+    acc = collections.defaultdict(list)
+    for r in rows:
+        if r["Counter_Name"] == counter:
+            acc[r["Kernel_Name"]].append((int(r["Grid_Size"]), float(r["Counter_Value"])))
+    return acc
+
+
+def _copy_reading(acc):
+    """Counter value of the largest copy dispatch (the 1 GiB device copy)."""
+    best = None
+    for k, v in acc.items():
+        if "copyBuffer" in k:
+            for grid, val in v:
+                if best is None or grid > best[0]:
+                    best = (grid, val)
+    if best is None:
+        raise RuntimeError("copy kernel not found in the counter data")
+    return best[1]
+
+
+def _summarize(args):
+    from walicxe3d_amd import workloads as W
+
+    w = W.get(args.workload)
+    zones = len(w.blocks()) * 4096
+    fetch = _per_kernel(_read(args.fetch), "FETCH_SIZE")
+    write = _per_kernel(_read(args.write), "WRITE_SIZE")
+    fscale = COPY_BYTES / (1024.0 * _copy_reading(fetch))
+    wscale = COPY_BYTES / (1024.0 * _copy_reading(write))
+    alg = {"stage1": zones * w.nv * (8 + 4), "stage2": zones * w.nv * (4 + 8 + 8)}
+    lines = ["# HBM traffic of the fused stage kernels, %s (%d blocks, %d zones, nv=%d)" %
+             (w.name, len(w.blocks()), zones, w.nv),
+             "# command: %s" % (args.cmd or "(see walicxe3d_amd/profile_run.py)"),
+             "# counters: FETCH_SIZE and WRITE_SIZE (KiB per dispatch), one rocprofv3 --pmc run each",
+             "# scale: the same counters on a %d-byte device copy (copyBuffer): bytes per counter byte" % COPY_BYTES,
+             "fetch_scale = %.4f" % fscale,
+             "write_scale = %.4f" % wscale]
+    out = {}
+    for k in fetch:
+        if "k_stage" not in k:
+            continue
+        st = "stage2" if "true" in k.split("<", 1)[1].split(",")[1] else "stage1"
+        f = sum(v for _, v in fetch[k]) / len(fetch[k])
+        wr = sum(v for _, v in write.get(k, [(0, 0.0)])) / max(1, len(write.get(k, [])))
+        hbm = 1024.0 * (f * fscale + wr * wscale)
+        out[st] = (k, f, wr, hbm)
+    for st in ("stage1", "stage2"):
+        if st not in out:
+            continue
+        k, f, wr, hbm = out[st]
+        lines += ["", "[%s] %s" % (st, k),
+                  "%s_fetch_kib = %.1f" % (st, f),
+                  "%s_write_kib = %.1f" % (st, wr),
+                  "%s_hbm_bytes_per_launch = %.4e" % (st, hbm),
+                  "%s_algorithmic_bytes_per_launch = %.4e" % (st, alg[st]),
+                  "%s_traffic_over_algorithmic = %.3f" % (st, hbm / alg[st])]
+    if "stage2" in out:
+        lines += ["", "hbm_bytes_per_launch = %.4e   (stage 2, the dominant kernel)" % out["stage2"][3]]
+    text = "\n".join(lines) + "\n"
+    if args.out:
+        with open(args.out, "w") as f:
+            f.write(text)
+    sys.stdout.write(text)
+    return 0
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    sub = ap.add_subparsers(dest="mode", required=True)
+    r = sub.add_parser("run")
+    r.add_argument("--workload", default="ot256")
+    r.add_argument("--reps", type=int, default=2)
+    s = sub.add_parser("summarize")
+    s.add_argument("--fetch", required=True)
+    s.add_argument("--write", required=True)
+    s.add_argument("--workload", default="ot256")
+    s.add_argument("--out")
+    s.add_argument("--cmd")
+    args = ap.parse_args(argv)
+    return _run(args) if args.mode == "run" else _summarize(args)
+
+
+if __name__ == "__main__":
+    sys.exit(main())
