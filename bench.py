@@ -96,6 +96,7 @@ def main():
     ap.add_argument("--err-steps", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="skip the launch autotuner (profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the block list of this path lives on a single device "
@@ -116,6 +117,15 @@ def main():
     # ---- fused FP32 form ----------------------------------------------------
     f = hip.Walicxe3D(w, form=hip.FUSED_FP32, bids=bids)
     f.upload(U0)
+    best = {"grid": 0, "xcd_map": 0}
+    if args.no_tune:
+        f.set_tuning(0, best["grid"])
+        f.set_tuning(1, best["xcd_map"])
+    else:
+        from walicxe3d_amd import autotune
+        best, table = autotune.tune(f, steps=3)
+        log("autotune: %s" % ", ".join("grid %d xcd %d %.3f ms" % (r["grid"], r["xcd_map"], r["ms"])
+                                       for r in table))
     f.run(args.warmup)
     t0 = time.perf_counter()
     f.run(args.steps)                      # returns after a stream synchronize
@@ -193,7 +203,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": "%s: %s" % (w.name, w.description), "blocks": nblk, "zones": zones,
                    "form": "fused FP32 bulk, FP64 conserved accumulation", "solver": w.solver,
-                   "limiter": w.limiter, "step_hbm_roofline_frac": (fused_zps * step_bytes / zones) / HBM_PEAK},
+                   "limiter": w.limiter, "step_hbm_roofline_frac": (fused_zps * step_bytes / zones) / HBM_PEAK,
+                   "tuning": {"grid": best["grid"], "xcd_map": best["xcd_map"]}},
         "roofline": roof,
         "cpu_baseline": cpu,
         "plain_fp64": plain,

@@ -26,6 +26,9 @@ import sys
 
 COPY_BYTES = 1 << 30
 
+if __package__ in (None, ""):   # run as a script: make the package importable
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+
 
 def _run(args):
     import numpy as np
