@@ -1,0 +1,54 @@
+// Test infrastructure: drives the library's C ABI (include/walicxe_hip.h) on
+// the CPU build of tests/emu for the cases named on the command line: each
+// pair is an input file written by tests/test_emulated_kernels.py and a step
+// count; the final state goes to <input>.out.
+#include <cstdio>
+#include <cstdlib>
+#include <vector>
+#include <cstring>
+#include <string>
+#include "walicxe_hip.h"
+static int run_case(const char *path, int nsteps);
+int main(int argc, char **argv) {
+  for (int a = 1; a + 1 < argc; a += 2) {
+    printf("case %s steps %s\n", argv[a], argv[a + 1]); fflush(stdout);
+    if (run_case(argv[a], atoi(argv[a + 1]))) return 1;
+  }
+  return 0;
+}
+static int run_case(const char *path, int nsteps) {
+  FILE *f = fopen(path, "rb");
+  int hdr[8];
+  fread(hdr, 4, 8, f);
+  int nv = hdr[0], nblk = hdr[1], ntab = hdr[2], nwind = hdr[3], maxlev = hdr[4];
+  int nroot[3] = {hdr[5], hdr[6], hdr[7]};
+  int clen; fread(&clen, 4, 1, f);
+  wx_config cfg; if (clen != (int)sizeof(cfg)) { printf("cfg size %d vs %zu\n", clen, sizeof(cfg)); return 1; }
+  fread(&cfg, clen, 1, f);
+  std::vector<double> tT(ntab), tC(ntab), wnd(8 * nwind), dx(3 * maxlev), U((size_t)nblk * nv * 4096);
+  std::vector<int> bids(nblk);
+  fread(tT.data(), 8, ntab, f); fread(tC.data(), 8, ntab, f); fread(wnd.data(), 8, 8 * nwind, f);
+  fread(dx.data(), 8, 3 * maxlev, f); fread(bids.data(), 4, nblk, f); fread(U.data(), 8, U.size(), f);
+  double dtout_tsc[2]; fread(dtout_tsc, 8, 2, f); fclose(f);
+  cfg.tab_logT = ntab ? tT.data() : nullptr; cfg.tab_loglam = ntab ? tC.data() : nullptr;
+  cfg.winds = nwind ? wnd.data() : nullptr;
+  void *ctx;
+  int rc = wx_init(&cfg, &ctx); if (rc < 0) { printf("init %d %s\n", rc, wx_last_error()); return 1; }
+  rc = wx_set_mesh(ctx, nroot, maxlev, dx.data(), nblk, bids.data()); if (rc < 0) { printf("mesh %d %s\n", rc, wx_last_error()); return 1; }
+  rc = wx_upload_state(ctx, U.data(), WX_LAYOUT_BLOCKS, 0, nullptr); if (rc < 0) { printf("upload %d %s\n", rc, wx_last_error()); return 1; }
+  int it = 0, nextout = 1; double dt = 0, time = 0;
+  if (cfg.form == WX_FORM_PLAIN_FP64) {
+    for (int n = 0; n < nsteps && rc >= 0; n++)
+      rc = wx_step(ctx, &it, &dt, &time, &nextout, dtout_tsc[0], dtout_tsc[1]);
+  } else {
+    rc = wx_run(ctx, nsteps, &it, &dt, &time, &nextout, dtout_tsc[0], dtout_tsc[1]);
+  }
+  if (rc < 0) { printf("run %d %s\n", rc, wx_last_error()); return 1; }
+  std::vector<double> Uo(U.size());
+  wx_download_state(ctx, Uo.data(), nullptr, WX_LAYOUT_BLOCKS, 0, nullptr);
+  { std::string fn = std::string(path) + ".out"; FILE *fo = fopen(fn.c_str(), "wb"); fwrite(Uo.data(), 8, Uo.size(), fo); fclose(fo); }
+  double s = 0; for (double x : Uo) s += x;
+  printf("it %d dt %.17g time %.17g sum %.17g\n", it, dt, time, s);
+  wx_finalize(ctx);
+  return 0;
+}

@@ -1,0 +1,106 @@
+"""The library's HIP sources compiled for the CPU (tests/emu) and run under
+AddressSanitizer: every kernel of the plain and fused forms executes on real
+workloads (AMR faces of every kind, cooling, winds, reflective / free-flow /
+periodic boundaries, hydro and MHD), and any out-of-range index into a device
+array aborts the run with its source line.  The emulation itself is checked
+against the oracle: the fused FP64 form must reproduce it bit for bit on
+Orszag-Tang, as it does on the GPU."""
+import os
+import shutil
+import struct
+import subprocess
+
+import numpy as np
+import pytest
+
+from walicxe3d_amd import hip
+from walicxe3d_amd import workloads as W
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+EMU = os.path.join(HERE, "emu")
+BUILD = os.path.join(EMU, "_build")
+CXX = shutil.which("g++")
+
+# (workload, form, steps)
+CASES = [("ot64", hip.FUSED_FP32, 2), ("sedov", hip.FUSED_FP32, 2), ("blast_reflect", hip.FUSED_FP32, 2),
+         ("amr_small", hip.FUSED_FP32, 2), ("mhd_hlle", hip.FUSED_FP32, 2), ("winds4", hip.FUSED_FP32, 1),
+         ("snr_amr4", hip.FUSED_FP32, 1), ("amr_small", hip.PLAIN_FP64, 1), ("mhd_hlle", hip.PLAIN_FP64, 1),
+         ("winds4", hip.PLAIN_FP64, 1)]
+
+
+def _build():
+    os.makedirs(BUILD, exist_ok=True)
+    exe = os.path.join(BUILD, "emu")
+    flags = ["-std=c++20", "-O1", "-g", "-fsanitize=address", "-fno-omit-frame-pointer", "-ffp-contract=off",
+             "-Wno-unknown-pragmas", "-I" + EMU, "-I" + os.path.join(ROOT, "walicxe3d_amd", "csrc"),
+             "-I" + os.path.join(ROOT, "include")]
+    objs = []
+    for src in ("wx_api", "wx_fused", "wx_plain"):
+        o = os.path.join(BUILD, src + ".o")
+        cmd = [CXX, "-x", "c++"] + flags + ["-c", os.path.join(ROOT, "walicxe3d_amd", "csrc", src + ".hip"),
+                                            "-o", o]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        assert r.returncode == 0, r.stderr[-3000:]
+        objs.append(o)
+    cmd = [CXX] + flags + [os.path.join(EMU, "emu_rt.cpp"), os.path.join(EMU, "emu_main.cpp")] + objs + ["-o", exe]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-3000:]
+    return exe
+
+
+def _write_input(path, w, form):
+    slots, blocks, U0 = W.initial_state(w)
+    cfg = hip.make_config(w, form)
+    raw = bytes(cfg)
+    lt, lam = W.cooling_table() if w.cooling else (np.zeros(0), np.zeros(0))
+    wp = np.asarray(W.wind_params(w), dtype=np.float64).reshape(-1)
+    m = w.mesh()
+    with open(path, "wb") as f:
+        f.write(struct.pack("<8i", w.nv, len(blocks), len(lt), len(wp) // 8, m.maxlev, *m.nroot))
+        f.write(struct.pack("<i", len(raw)))
+        f.write(raw)
+        for a in (lt, lam, wp, np.array(m.dx, dtype=np.float64).reshape(-1)):
+            f.write(np.asarray(a, dtype="<f8").tobytes())
+        f.write(np.asarray(blocks, dtype="<i4").tobytes())
+        f.write(np.ascontiguousarray(U0, dtype="<f8").tobytes())
+        f.write(struct.pack("<2d", w.dtout, w.t_sc))
+    return blocks, U0
+
+
+@pytest.fixture(scope="module")
+def emu():
+    if CXX is None:
+        pytest.skip("g++ not available")
+    return _build()
+
+
+def _run(emu, args, tmp):
+    env = dict(os.environ, ASAN_OPTIONS="detect_stack_use_after_return=0:detect_leaks=0")
+    r = subprocess.run([emu] + args, capture_output=True, text=True, timeout=900, env=env, cwd=tmp)
+    assert r.returncode == 0 and "ERROR: AddressSanitizer" not in r.stderr, (r.stdout + r.stderr)[-4000:]
+    return r.stdout
+
+
+def test_every_kernel_in_bounds(emu, tmp_path):
+    args = []
+    for n, (name, form, steps) in enumerate(CASES):
+        p = str(tmp_path / ("in%d.bin" % n))
+        _write_input(p, W.get(name), form)
+        args += [p, str(steps)]
+    out = _run(emu, args, str(tmp_path))
+    assert out.count("it ") == len(CASES), out
+
+
+def test_emulation_reproduces_fused_fp64(emu, tmp_path, oracle):
+    w = W.get("ot64")
+    p = str(tmp_path / "ot64.bin")
+    blocks, U0 = _write_input(p, w, hip.FUSED_FP64)
+    _run(emu, [p, "3"], str(tmp_path))
+    o = oracle.Sim(w, blocks)
+    o.set_state(U0)
+    for _ in range(3):
+        o.step()
+    uo, _ = o.get_state()
+    ue = np.fromfile(p + ".out").reshape(uo.shape)
+    assert np.array_equal(ue, uo)
