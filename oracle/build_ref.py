@@ -66,7 +66,7 @@ def flit(x):
     return s
 
 
-def parameters_text(w):
+def parameters_text(w, nprocs=1):
     nblk_max = w.max_blocks_ref()
     nv = (8 if w.mhd else 5) + 1
     block_ram = 20 ** 3 * nv * 8
@@ -74,7 +74,7 @@ def parameters_text(w):
     ram_mb = int(((nblk_max + 2) * block_ram * 3 / 0.95 + 6 * block_ram) / (1024 * 1024)) + 2
     bcs = w.bc
     sub = {
-        "TFIN": flit(w.tfin), "DTOUT": flit(w.dtout), "RAM_MB": flit(float(ram_mb)),
+        "NPROCS": str(nprocs), "TFIN": flit(w.tfin), "DTOUT": flit(w.dtout), "RAM_MB": flit(float(ram_mb)),
         "XPHYS": flit(w.phys[0]), "YPHYS": flit(w.phys[1]), "ZPHYS": flit(w.phys[2]),
         "NBX": str(w.nroot[0]), "NBY": str(w.nroot[1]), "NBZ": str(w.nroot[2]),
         "MAXLEV": str(w.maxlev),
@@ -120,12 +120,12 @@ def gpu_sources():
     return out
 
 
-def build(name, gpu=False):
+def build(name, gpu=False, nprocs=1):
     w = workloads.get(name)
-    bdir = os.path.join(OUT, name + ("_gpu" if gpu else ""))
+    bdir = os.path.join(OUT, name + ("_gpu" if gpu else "") + ("_np%d" % nprocs if nprocs > 1 else ""))
     os.makedirs(bdir, exist_ok=True)
     with open(os.path.join(bdir, "parameters.f90"), "w") as f:
-        f.write(parameters_text(w))
+        f.write(parameters_text(w, nprocs))
     for fn in ("user.f90", "harness.f90"):
         shutil.copyfile(os.path.join(HERE, "ref", fn), os.path.join(bdir, fn))
     flags = ["-cpp", "-DMPIP", "-DDOUBLEP", "-fdefault-real-8", os.environ.get("WX_FOPT", "-O2"),
@@ -166,18 +166,23 @@ def main(argv):
     gpu = "--gpu" in argv
     argv = [a for a in argv if not a.startswith("--")]
     names = argv or workloads.oracle_builds()
-    jobs = [(n, False) for n in names]
+    jobs = [(n, False, 1) for n in names]
     if gpu or not argv:
         # drop-in builds used by tests/test_gpu_fortran.py
-        jobs += [(n, True) for n in (argv or DROPIN_WORKLOADS)]
+        jobs += [(n, True, 1) for n in (argv or DROPIN_WORKLOADS)]
+    if not argv or "--baseline" in sys.argv:
+        # multi-rank builds for the CPU baseline of bench.py (mpiexec -n N)
+        jobs += [(n, False, np_) for n, np_ in BASELINE_BUILDS]
     from concurrent.futures import ThreadPoolExecutor
     with ThreadPoolExecutor(max_workers=4) as ex:
-        for (n, g), exe in zip(jobs, ex.map(lambda j: build(*j), jobs)):
-            print("building", n, "(gpu drop-in)" if g else "", "->", exe)
+        for (n, g, np_), exe in zip(jobs, ex.map(lambda j: build(*j), jobs)):
+            print("building", n, "(gpu drop-in)" if g else "", "(%d ranks)" % np_ if np_ > 1 else "", "->", exe)
     return 0
 
 
 DROPIN_WORKLOADS = ["sedov", "ot64", "amr_small", "blast_reflect"]
+# (workload, MPI ranks): 16 ranks = the host cores one GPU of the box comes with
+BASELINE_BUILDS = [("ot64", 16)]
 
 
 if __name__ == "__main__":

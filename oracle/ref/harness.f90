@@ -64,27 +64,31 @@ program wx_oracle_harness
   call updatePrims ()
   if (nextout == 0) nextout = 1
 
-  call write_topology ()
-  open (unit=71, file='steps.txt', status='replace', action='write')
-  if (any(dumps(1:max(ndump,1)) == 0)) call write_state (0)
+  ! multi-rank runs (timing): the master writes the step log only
+  if (rank == master) then
+    if (nProcs == 1) call write_topology ()
+    open (unit=71, file='steps.txt', status='replace', action='write')
+  end if
+  if (nProcs == 1 .and. any(dumps(1:max(ndump,1)) == 0)) call write_state (0)
 
   do s=1,nsteps
     it = it + 1
     call hydroSolver ()
     call updatePrims ()
     call cooling ()
-    write (71,'(i8,3es26.17e3)') it, dt, time, nextout*dtout/t_sc
+    if (rank == master) write (71,'(i8,3es26.17e3)') it, dt, time, nextout*dtout/t_sc
     if (dumpout) then
       nextout = nextout + 1
       dumpout = .false.
     end if
-    if (any(dumps(1:max(ndump,1)) == s)) call write_state (s)
+    if (nProcs == 1 .and. any(dumps(1:max(ndump,1)) == s)) call write_state (s)
     if (tstop > 0.0 .and. time >= tstop) then
-      if (.not. any(dumps(1:max(ndump,1)) == s)) call write_state (s)
+      if (nProcs == 1 .and. .not. any(dumps(1:max(ndump,1)) == s)) call write_state (s)
       exit
     end if
   end do
-  close (71)
+  if (rank == master) close (71)
+  call mpi_finalize (ierr)
 
 contains
 

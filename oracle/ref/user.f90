@@ -12,9 +12,15 @@
 !                                 interior conserved state of every block
 !   userBoundary(uvars)         - periodic faces (the reference leaves
 !                                 BC_PERIODIC ghost cells unfilled,
-!                                 source/boundary.f90:594) and spherical wind
-!                                 sources through the reference's own
-!                                 imposeSphericalWind (source/winds.f90:123)
+!                                 source/boundary.f90:594), across MPI ranks
+!                                 when the partner block lives elsewhere, and
+!                                 spherical wind sources through the
+!                                 reference's own imposeSphericalWind
+!                                 (source/winds.f90:123)
+!
+! Multi-rank runs (nProcs > 1): the root blocks start on the master
+! (source/init.f90:766-786), so the master refines and fills every block and
+! the harness's loadBalance then spreads them over the ranks.
 !   get_user_source_terms       - no user sources
 !
 ! File layout (little endian, stream):
@@ -68,9 +74,11 @@ contains
 
     allocate (reflist(max(nref,1)))
     if (nref > 0) read (77) reflist(1:nref)
-    do n=1,nref
-      call refineBlock (reflist(n))
-    end do
+    if (rank == master) then
+      do n=1,nref
+        call refineBlock (reflist(n))
+      end do
+    end if
     call syncBlockLists ()
 
     nwinds = nw
@@ -95,6 +103,7 @@ contains
       read (77) buf
       call find (int(bid), localBlocks, nbMaxProc, slot)
       if (slot == -1) then
+        if (rank /= master) cycle
         write(*,*) 'ic.bin block not present in the mesh:', bid
         stop 4
       end if
@@ -129,70 +138,184 @@ contains
 
   !-----------------------------------------------------------------------------
   ! Fills two ghost layers on every periodic domain face from the block that
-  ! wraps around at the same level (single-rank runs).
+  ! wraps around at the same level.  A partner on another rank exchanges its
+  ! two near-face layers by non-blocking messages (tag: global index of the
+  ! receiving block x 6 + face).
   subroutine periodicFill (uvars)
 
     use amr, only : meshlevel, bcoords, offsets
     implicit none
     real, intent(inout) :: uvars (nbMaxProc, neqtot, &
                            nxmin:nxmax, nymin:nymax, nzmin:nzmax)
-    integer :: nb, bID, lev, x, y, z, nx, ny, nz, face, sID, s, g
-    integer :: xs, ys, zs
+    integer, parameter :: nslab = 2*ncells_block*ncells_block
+    integer :: nb, bID, face, sID, s, g, gidx, owner, ex, nex, ierr2
+    integer :: ibase, recvface(6*nbMaxProc), recvblk(6*nbMaxProc)
+    integer :: req(2*6*nbMaxProc)
+    real, allocatable :: sbuf(:,:,:,:,:), rbuf(:,:,:,:,:)
 
+    ! count the faces whose partner lives on another rank
+    nex = 0
     do nb=1,nbMaxProc
       bID = localBlocks(nb)
       if (bID == -1) cycle
-      call meshlevel (bID, lev)
-      call bcoords (bID, x, y, z)
-      nx = nbx(lev); ny = nby(lev); nz = nbz(lev)
       do face=1,6
-        xs = x; ys = y; zs = z
-        select case (face)
-        case (LEFT)
-          if (bc_left /= BC_PERIODIC .or. x /= 1) cycle
-          xs = nx
-        case (RIGHT)
-          if (bc_right /= BC_PERIODIC .or. x /= nx) cycle
-          xs = 1
-        case (FRONT)
-          if (bc_front /= BC_PERIODIC .or. y /= 1) cycle
-          ys = ny
-        case (BACK)
-          if (bc_back /= BC_PERIODIC .or. y /= ny) cycle
-          ys = 1
-        case (BOTTOM)
-          if (bc_bottom /= BC_PERIODIC .or. z /= 1) cycle
-          zs = nz
-        case (TOP)
-          if (bc_top /= BC_PERIODIC .or. z /= nz) cycle
-          zs = 1
-        end select
-        sID = 1 + (xs-1) + (ys-1)*nx + (zs-1)*nx*ny + offsets(lev)
-        call find (sID, localBlocks, nbMaxProc, s)
-        if (s == -1) then
+        call partner (bID, face, sID)
+        if (sID == -1) cycle
+        call find (sID, globalBlocks, nbMaxGlobal, gidx)
+        if (gidx /= -1) then
+          if ((gidx - 1) / nbMaxProc /= rank) nex = nex + 1
+        end if
+      end do
+    end do
+    allocate (sbuf(neqtot, ncells_block, ncells_block, 2, max(nex,1)))
+    allocate (rbuf(neqtot, ncells_block, ncells_block, 2, max(nex,1)))
+    nex = 0
+    do nb=1,nbMaxProc
+      bID = localBlocks(nb)
+      if (bID == -1) cycle
+      do face=1,6
+        call partner (bID, face, sID)
+        if (sID == -1) cycle
+        call find (sID, globalBlocks, nbMaxGlobal, gidx)
+        if (gidx == -1) then
           write(*,*) 'periodic partner not on the same level:', bID, sID
           stop 5
         end if
-        do g=1,2
-          select case (face)
-          case (LEFT)
-            uvars(nb,:,1-g,1:ncells_y,1:ncells_z) = uvars(s,:,ncells_x+1-g,1:ncells_y,1:ncells_z)
-          case (RIGHT)
-            uvars(nb,:,ncells_x+g,1:ncells_y,1:ncells_z) = uvars(s,:,g,1:ncells_y,1:ncells_z)
-          case (FRONT)
-            uvars(nb,:,1:ncells_x,1-g,1:ncells_z) = uvars(s,:,1:ncells_x,ncells_y+1-g,1:ncells_z)
-          case (BACK)
-            uvars(nb,:,1:ncells_x,ncells_y+g,1:ncells_z) = uvars(s,:,1:ncells_x,g,1:ncells_z)
-          case (BOTTOM)
-            uvars(nb,:,1:ncells_x,1:ncells_y,1-g) = uvars(s,:,1:ncells_x,1:ncells_y,ncells_z+1-g)
-          case (TOP)
-            uvars(nb,:,1:ncells_x,1:ncells_y,ncells_z+g) = uvars(s,:,1:ncells_x,1:ncells_y,g)
-          end select
-        end do
+        owner = (gidx - 1) / nbMaxProc
+        if (owner == rank) then
+          s = gidx - owner*nbMaxProc
+          do g=1,2
+            call copy_layer (uvars, nb, face, g, s)
+          end do
+        else
+          nex = nex + 1
+          recvface(nex) = face
+          recvblk(nex) = nb
+          ! this block's own layers, for the partner's opposite face
+          do g=1,2
+            call pack_layer (uvars, nb, face, g, sbuf(:,:,:,g,nex))
+          end do
+          call find (bID, globalBlocks, nbMaxGlobal, ibase)
+          call mpi_irecv (rbuf(1,1,1,1,nex), neqtot*nslab, mpi_real_kind, owner, &
+                          ibase*6 + face - 1, mpi_comm_world, req(2*nex-1), ierr2)
+          call mpi_isend (sbuf(1,1,1,1,nex), neqtot*nslab, mpi_real_kind, owner, &
+                          gidx*6 + opposite(face) - 1, mpi_comm_world, req(2*nex), ierr2)
+        end if
       end do
     end do
+    if (nex > 0) then
+      call mpi_waitall (2*nex, req, MPI_STATUSES_IGNORE, ierr2)
+      do ex=1,nex
+        do g=1,2
+          call unpack_layer (uvars, recvblk(ex), recvface(ex), g, rbuf(:,:,:,g,ex))
+        end do
+      end do
+    end if
+    deallocate (sbuf, rbuf)
 
   end subroutine periodicFill
+
+  integer function opposite (face)
+    integer, intent(in) :: face
+    opposite = face + 1 - 2*mod(face+1, 2)
+  end function opposite
+
+  !> Same-level block across a periodic domain face of bID, or -1.
+  subroutine partner (bID, face, sID)
+    use amr, only : meshlevel, bcoords, offsets
+    integer, intent(in) :: bID, face
+    integer, intent(out) :: sID
+    integer :: lev, x, y, z, nx, ny, nz, xs, ys, zs
+    call meshlevel (bID, lev)
+    call bcoords (bID, x, y, z)
+    nx = nbx(lev); ny = nby(lev); nz = nbz(lev)
+    xs = x; ys = y; zs = z
+    sID = -1
+    select case (face)
+    case (LEFT)
+      if (bc_left /= BC_PERIODIC .or. x /= 1) return
+      xs = nx
+    case (RIGHT)
+      if (bc_right /= BC_PERIODIC .or. x /= nx) return
+      xs = 1
+    case (FRONT)
+      if (bc_front /= BC_PERIODIC .or. y /= 1) return
+      ys = ny
+    case (BACK)
+      if (bc_back /= BC_PERIODIC .or. y /= ny) return
+      ys = 1
+    case (BOTTOM)
+      if (bc_bottom /= BC_PERIODIC .or. z /= 1) return
+      zs = nz
+    case (TOP)
+      if (bc_top /= BC_PERIODIC .or. z /= nz) return
+      zs = 1
+    end select
+    sID = 1 + (xs-1) + (ys-1)*nx + (zs-1)*nx*ny + offsets(lev)
+  end subroutine partner
+
+  !> Ghost layer g of block nb at face from the wrapped block s (same rank).
+  subroutine copy_layer (uvars, nb, face, g, s)
+    real, intent(inout) :: uvars (nbMaxProc, neqtot, nxmin:nxmax, nymin:nymax, nzmin:nzmax)
+    integer, intent(in) :: nb, face, g, s
+    select case (face)
+    case (LEFT)
+      uvars(nb,:,1-g,1:ncells_y,1:ncells_z) = uvars(s,:,ncells_x+1-g,1:ncells_y,1:ncells_z)
+    case (RIGHT)
+      uvars(nb,:,ncells_x+g,1:ncells_y,1:ncells_z) = uvars(s,:,g,1:ncells_y,1:ncells_z)
+    case (FRONT)
+      uvars(nb,:,1:ncells_x,1-g,1:ncells_z) = uvars(s,:,1:ncells_x,ncells_y+1-g,1:ncells_z)
+    case (BACK)
+      uvars(nb,:,1:ncells_x,ncells_y+g,1:ncells_z) = uvars(s,:,1:ncells_x,g,1:ncells_z)
+    case (BOTTOM)
+      uvars(nb,:,1:ncells_x,1:ncells_y,1-g) = uvars(s,:,1:ncells_x,1:ncells_y,ncells_z+1-g)
+    case (TOP)
+      uvars(nb,:,1:ncells_x,1:ncells_y,ncells_z+g) = uvars(s,:,1:ncells_x,1:ncells_y,g)
+    end select
+  end subroutine copy_layer
+
+  !> Interior layer g of block nb that the partner across `face` uses as its
+  !! ghost layer g (layer g from the face: g for low faces, n+1-g for high).
+  subroutine pack_layer (uvars, nb, face, g, buf)
+    real, intent(in) :: uvars (nbMaxProc, neqtot, nxmin:nxmax, nymin:nymax, nzmin:nzmax)
+    integer, intent(in) :: nb, face, g
+    real, intent(out) :: buf (neqtot, ncells_block, ncells_block)
+    select case (face)
+    case (LEFT)
+      buf = uvars(nb,:,g,1:ncells_y,1:ncells_z)
+    case (RIGHT)
+      buf = uvars(nb,:,ncells_x+1-g,1:ncells_y,1:ncells_z)
+    case (FRONT)
+      buf = uvars(nb,:,1:ncells_x,g,1:ncells_z)
+    case (BACK)
+      buf = uvars(nb,:,1:ncells_x,ncells_y+1-g,1:ncells_z)
+    case (BOTTOM)
+      buf = uvars(nb,:,1:ncells_x,1:ncells_y,g)
+    case (TOP)
+      buf = uvars(nb,:,1:ncells_x,1:ncells_y,ncells_z+1-g)
+    end select
+  end subroutine pack_layer
+
+  !> Ghost layer g of block nb at `face` from the partner's packed layer g.
+  subroutine unpack_layer (uvars, nb, face, g, buf)
+    real, intent(inout) :: uvars (nbMaxProc, neqtot, nxmin:nxmax, nymin:nymax, nzmin:nzmax)
+    integer, intent(in) :: nb, face, g
+    real, intent(in) :: buf (neqtot, ncells_block, ncells_block)
+    select case (face)
+    case (LEFT)
+      uvars(nb,:,1-g,1:ncells_y,1:ncells_z) = buf
+    case (RIGHT)
+      uvars(nb,:,ncells_x+g,1:ncells_y,1:ncells_z) = buf
+    case (FRONT)
+      uvars(nb,:,1:ncells_x,1-g,1:ncells_z) = buf
+    case (BACK)
+      uvars(nb,:,1:ncells_x,ncells_y+g,1:ncells_z) = buf
+    case (BOTTOM)
+      uvars(nb,:,1:ncells_x,1:ncells_y,1-g) = buf
+    case (TOP)
+      uvars(nb,:,1:ncells_x,1:ncells_y,ncells_z+g) = buf
+    end select
+  end subroutine unpack_layer
 
   !-----------------------------------------------------------------------------
   subroutine get_user_source_terms (pp, s, i, j, k)

@@ -15,6 +15,7 @@ from walicxe3d_amd import workloads as W
 
 ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
 REFDIR = os.path.join(ROOT, "oracle", "_ref")
+MPIEXEC = "/opt/conda/bin/mpiexec"
 
 
 def harness_path(name: str) -> str:
@@ -55,9 +56,14 @@ def read_state(path, nblk, nv_ref):
     return a.reshape(nblk, nv_ref, 16, 16, 16)
 
 
-def run(w, nsteps, dumps, workdir=None, unit_op=None, unit_in=None, timeout=3600, tstop=-1.0, variant=""):
+def run(w, nsteps, dumps, workdir=None, unit_op=None, unit_in=None, timeout=3600, tstop=-1.0, variant="",
+        nprocs=1):
     """Run the harness; returns dict with topo, steps (it, dt, time, tout),
-    and states {step: (U, PRIM)} with the passive scalar stripped."""
+    and states {step: (U, PRIM)} with the passive scalar stripped.  With
+    nprocs > 1 the multi-rank build runs under mpiexec and only the step log
+    comes back (no topology, no states)."""
+    if nprocs > 1:
+        variant = variant + "_np%d" % nprocs
     exe = harness_path(w.name + variant)
     if not os.path.exists(exe):
         raise FileNotFoundError(exe)
@@ -80,9 +86,16 @@ def run(w, nsteps, dumps, workdir=None, unit_op=None, unit_in=None, timeout=3600
     with open(os.path.join(wd, "harness.nml"), "w") as f:
         f.write("&ctl\n mode='run'\n nsteps=%d\n ndump=%d\n dumps=%s\n tstop=%r\n/\n"
                 % (nsteps, len(dumps), dl or "-1", float(tstop)))
-    r = subprocess.run([exe], cwd=wd, timeout=timeout, capture_output=True, text=True)
+    cmd = [exe] if nprocs == 1 else [MPIEXEC, "-n", str(nprocs), exe]
+    r = subprocess.run(cmd, cwd=wd, timeout=timeout, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("reference harness failed (%d): %s %s" % (r.returncode, r.stdout[-2000:], r.stderr[-2000:]))
+    if nprocs > 1:
+        steps = np.loadtxt(os.path.join(wd, "steps.txt"), ndmin=2)
+        if own:
+            import shutil
+            shutil.rmtree(wd, ignore_errors=True)
+        return dict(steps=steps)
     topo = read_topology(os.path.join(wd, "topo.bin"))
     steps = np.loadtxt(os.path.join(wd, "steps.txt"), ndmin=2)
     nblk = len(topo["bids"])
