@@ -216,8 +216,9 @@ def drift(U0, u):
     relative to the L1 size of that quantity (momentum and field measured by
     the magnitude of the vector, so that a near-zero component such as the
     z-momentum of Orszag-Tang is judged on the momentum scale)."""
-    t0 = U0.sum(axis=(0, 2, 3, 4))
-    t1 = u.sum(axis=(0, 2, 3, 4))
+    # totals in extended precision: a float64 sum over 10^7 cells errs by ~1e-13
+    t0 = U0.sum(axis=(0, 2, 3, 4), dtype=np.longdouble)
+    t1 = u.sum(axis=(0, 2, 3, 4), dtype=np.longdouble)
     mag_m = np.sqrt((U0[:, 1:4] ** 2).sum(axis=1)).sum()
     scale = np.empty(U0.shape[1])
     scale[0] = np.abs(U0[:, 0]).sum()
@@ -225,7 +226,7 @@ def drift(U0, u):
     scale[4] = np.abs(U0[:, 4]).sum()
     if U0.shape[1] == 8:
         scale[5:8] = np.sqrt((U0[:, 5:8] ** 2).sum(axis=1)).sum()
-    return np.abs(t1 - t0) / scale
+    return np.abs(np.asarray(t1 - t0, dtype=np.float64)) / scale
 
 
 @pytest.mark.parametrize("form", ["fused_fp32", "fused_fp64", "plain"])

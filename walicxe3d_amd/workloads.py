@@ -326,6 +326,15 @@ _reg(_w(name="winds4", mhd=False, solver="hll", limiter="minmod", gamma_expr="5.
         nroot=(1, 1, 1), maxlev=5, base_level=2, phys=(10.0 * PC, 10.0 * PC, 10.0 * PC), ic="winds", seed=5,
         nsteps=5, cooling=True, refine_rule="winds", **_PHYS_UNITS,
         description="three spherical wind sources on 4 AMR levels, cooling, HLL"))
+# C5 full size (GPU only): winds, root 8^3 at level 4, 5 levels (4..8), ~32k blocks; HLL and HLLC
+for _solver, _suffix in (("hll", ""), ("hllc", "_hllc")):
+    _reg(_w(name="winds32k" + _suffix, mhd=False, solver=_solver, limiter="minmod", gamma_expr="5.0/3.0",
+            gamma=5.0 / 3.0, cfl=0.4, visc_eta=5.0e-3, eight_wave=False, bc=FREE6,
+            nroot=(1, 1, 1), maxlev=8, base_level=4, phys=(10.0 * PC, 10.0 * PC, 10.0 * PC), ic="winds", seed=5,
+            nsteps=5, cooling=True, refine_rule="winds", refine_radii=(0.5, 0.27, 0.13, 0.065), oracle=False,
+            **_PHYS_UNITS,
+            description="three spherical wind sources on 5 AMR levels (root 8^3), 32,278 blocks, cooling, %s"
+                        % _solver.upper()))
 
 
 def get(name: str) -> Workload:
