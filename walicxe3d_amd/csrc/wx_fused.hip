@@ -243,7 +243,11 @@ __global__ void __launch_bounds__(256, WX_STAGE_MIN_WAVES) k_stage(FusedArgs a) 
   constexpr int NXF = (NC + 1) * NC;           // x (or y) interfaces per plane
   __shared__ RF sp[NV][W * WP];                 // primitives of the plane (cross region)
   __shared__ RF sfx[NV][NC * (NC + 1)];        // x fluxes [v][(j-1)*17 + s]
-  __shared__ RF sfy[NV][(NC + 1) * NC];        // y fluxes [v][s*16 + (i-1)]
+  __shared__ RF sfy[NV][(NC + 1) * NC];        // y fluxes, see YT
+  // y-flux layout: YT (MHD kernels) stores [v][(i-1)*17 + s] like sfx, so the
+  // stores of consecutive lanes (consecutive s) and the update's reads (lanes
+  // along i, row pitch 17) hit distinct LDS banks; otherwise [v][s*16 + (i-1)]
+  constexpr bool YT = NV == 8;
   __shared__ RF sun[SV ? 2 : 1][SV ? NV : 1][NC + 2][NC + 2];  // UP_new planes + half-step ghost ring
   __shared__ double sred[4];
 
@@ -385,7 +389,7 @@ __global__ void __launch_bounds__(256, WX_STAGE_MIN_WAVES) k_stage(FusedArgs a) 
           for (int v = 0; v < NV; v++) sfx[v][aa * (NC + 1) + s] = ff[v];
         } else {
 #pragma unroll
-          for (int v = 0; v < NV; v++) sfy[v][s * NC + aa] = ff[v];
+          for (int v = 0; v < NV; v++) sfy[v][YT ? aa * (NC + 1) + s : s * NC + aa] = ff[v];
         }
       }
       __syncthreads();
@@ -407,7 +411,8 @@ __global__ void __launch_bounds__(256, WX_STAGE_MIN_WAVES) k_stage(FusedArgs a) 
 #pragma unroll
         for (int v = 0; v < NV; v++)
           unew[v] = ub[v] + dtdx * ((double)sfx[v][tj * (NC + 1) + ti] - (double)sfx[v][tj * (NC + 1) + ti + 1]) +
-                    dtdy * ((double)sfy[v][tj * NC + ti] - (double)sfy[v][(tj + 1) * NC + ti]) +
+                    dtdy * ((double)sfy[v][YT ? ti * (NC + 1) + tj : tj * NC + ti] -
+                            (double)sfy[v][YT ? ti * (NC + 1) + tj + 1 : (tj + 1) * NC + ti]) +
                     dtdz * ((double)fzp[v] - (double)fzc[v]);
         if (NV == 8 && c.eight_wave) {
           // Powell source (source/sources.f90:38-95) from the stage primitives

@@ -32,6 +32,20 @@ __host__ __device__ inline float rfmax(float a, float b) { return fmaxf(a, b); }
 __host__ __device__ inline double rcopysign(double a, double b) { return copysign(a, b); }
 __host__ __device__ inline float rcopysign(float a, float b) { return copysignf(a, b); }
 
+// Division in the MHD wave-speed and HLLD code.  FP64: IEEE division (the
+// plain form's bit-exact arithmetic).  FP32 on the device: the hardware
+// reciprocal (about 1 ulp) times the numerator, two instructions instead of
+// the range-scaled sequence; the FP32 form's fluxes carry FP32 rounding
+// anyway.  On the host (tests) FP32 division stays IEEE.
+__host__ __device__ inline double rdiv(double a, double b) { return a / b; }
+__host__ __device__ inline float rdiv(float a, float b) {
+#ifdef __HIP_DEVICE_COMPILE__
+  return a * __builtin_amdgcn_rcpf(b);
+#else
+  return a / b;
+#endif
+}
+
 // flow2prim (source/hydro_core.f90:323-378); returns istat (0, 1 p floor, 2 rho floor)
 template <typename R>
 __host__ __device__ inline int flow2prim(const Phys<R> &c, const R *u, R *p) {
@@ -107,15 +121,15 @@ template <typename R>
 __host__ __device__ inline R cfast1(const Phys<R> &c, R p, R d, R bx, R by, R bz, R bd) {
   R b2 = bx * bx + by * by + bz * bz;
   R a = c.gamma * p + b2;
-  return rsqrt_(R(0.5) * (a + rsqrt_(a * a - R(4.0) * c.gamma * p * bd * bd)) / d);
+  return rsqrt_(rdiv(R(0.5) * (a + rsqrt_(a * a - R(4.0) * c.gamma * p * bd * bd)), d));
 }
 
 // cfastX (source/hydro_core.f90:553-566)
 template <typename R> __host__ __device__ inline R cfastX(const Phys<R> &c, const R *p) {
   R b2 = p[5] * p[5] + p[6] * p[6] + p[7] * p[7];
-  R cs2va2 = (c.gamma * p[4] + b2) / p[0];
+  R cs2va2 = rdiv(c.gamma * p[4] + b2, p[0]);
   return rsqrt_(R(0.5) *
-                (cs2va2 + rsqrt_(cs2va2 * cs2va2 - R(4.0) * c.gamma * p[4] * (p[5] * p[5]) / p[0] / p[0])));
+                (cs2va2 + rsqrt_(cs2va2 * cs2va2 - rdiv(rdiv(R(4.0) * c.gamma * p[4] * (p[5] * p[5]), p[0]), p[0]))));
 }
 
 // average (source/hydro_core.f90:640-697)
@@ -239,10 +253,10 @@ __host__ __device__ inline void hlld_star(const R *p, R bx, R smuK, R smsM, R sM
   if (den == R(0)) {
     vst = p[2]; wst = p[3]; byst = R(0); bzst = R(0);
   } else {
-    vst = p[2] - bx * p[6] * sMmuK / den;
-    wst = p[3] - bx * p[7] * sMmuK / den;
-    byst = p[6] * (p[0] * (smuK * smuK) - bx * bx) / den;
-    bzst = p[7] * (p[0] * (smuK * smuK) - bx * bx) / den;
+    vst = p[2] - rdiv(bx * p[6] * sMmuK, den);
+    wst = p[3] - rdiv(bx * p[7] * sMmuK, den);
+    byst = rdiv(p[6] * (p[0] * (smuK * smuK) - bx * bx), den);
+    bzst = rdiv(p[7] * (p[0] * (smuK * smuK) - bx * bx), den);
   }
 }
 
@@ -263,15 +277,15 @@ __host__ __device__ inline void primfhlld(const Phys<R> &c, const R *pl, const R
   R srmur = sr - pr[1];
   R rholul = pl[0] * pl[1];
   R rhorur = pr[0] * pr[1];
-  R sM = (srmur * rhorur - slmul * rholul - pTR + pTL) / (srmur * pr[0] - slmul * pl[0]);
+  R sM = rdiv(srmur * rhorur - slmul * rholul - pTR + pTL, srmur * pr[0] - slmul * pl[0]);
   R srmsM = sr - sM;
   R slmsM = sl - sM;
-  R rhostl = pl[0] * slmul / slmsM;
-  R rhostr = pr[0] * srmur / srmsM;
-  R sstl = sM - rabs(bx) / rsqrt_(rhostl);
-  R sstr = sM + rabs(bx) / rsqrt_(rhostr);
-  R pst = (srmur * pr[0] * pTL - slmul * pl[0] * pTR + pl[0] * pr[0] * srmur * slmul * (pr[1] - pl[1])) /
-          (srmur * pr[0] - slmul * pl[0]);
+  R rhostl = rdiv(pl[0] * slmul, slmsM);
+  R rhostr = rdiv(pr[0] * srmur, srmsM);
+  R sstl = sM - rdiv(rabs(bx), rsqrt_(rhostl));
+  R sstr = sM + rdiv(rabs(bx), rsqrt_(rhostr));
+  R pst = rdiv(srmur * pr[0] * pTL - slmul * pl[0] * pTR + pl[0] * pr[0] * srmur * slmul * (pr[1] - pl[1]),
+               srmur * pr[0] - slmul * pl[0]);
   R vstl, wstl, bystl, bzstl, vstr, wstr, bystr, bzstr;
   if (sstl >= R(0)) {  // UL*
     R el = R(0.5) * pl[0] * (pl[1] * pl[1] + pl[2] * pl[2] + pl[3] * pl[3]) + cv * pl[4] +
@@ -279,7 +293,7 @@ __host__ __device__ inline void primfhlld(const Phys<R> &c, const R *pl, const R
     hlld_star(pl, bx, slmul, slmsM, sM - pl[1], vstl, wstl, bystl, bzstl);
     R vdotbl = pl[1] * bx + pl[2] * pl[6] + pl[3] * pl[7];
     R vstdotbstl = sM * bx + vstl * bystl + wstl * bzstl;
-    R estl = (slmul * el - pTL * pl[1] + pst * sM + bx * (vdotbl - vstdotbstl)) / slmsM;
+    R estl = rdiv(slmul * el - pTL * pl[1] + pst * sM + bx * (vdotbl - vstdotbstl), slmsM);
     ff[0] = rhostl * sM;
     ff[1] = rhostl * (sM * sM) + pst - bx * bx;
     ff[2] = rhostl * sM * vstl - bx * bystl;
@@ -296,7 +310,7 @@ __host__ __device__ inline void primfhlld(const Phys<R> &c, const R *pl, const R
     hlld_star(pr, bx, srmur, srmsM, sM - pr[1], vstr, wstr, bystr, bzstr);
     R vdotbr = pr[1] * bx + pr[2] * pr[6] + pr[3] * pr[7];
     R vstdotbstr = sM * bx + vstr * bystr + wstr * bzstr;
-    R estr = (srmur * er - pTR * pr[1] + pst * sM + bx * (vdotbr - vstdotbstr)) / srmsM;
+    R estr = rdiv(srmur * er - pTR * pr[1] + pst * sM + bx * (vdotbr - vstdotbstr), srmsM);
     ff[0] = rhostr * sM;
     ff[1] = rhostr * (sM * sM) + pst - bx * bx;
     ff[2] = rhostr * sM * vstr - bx * bystr;
@@ -311,18 +325,18 @@ __host__ __device__ inline void primfhlld(const Phys<R> &c, const R *pl, const R
   hlld_star(pr, bx, srmur, srmsM, sM - pr[1], vstr, wstr, bystr, bzstr);
   R sql = rsqrt_(rhostl), sqr = rsqrt_(rhostr);
   R dd = sql + sqr;
-  R vstst = (sql * vstl + sqr * vstr + (bystr - bystl) * signBx) / dd;
-  R wstst = (sql * wstl + sqr * wstr + (bzstr - bzstl) * signBx) / dd;
+  R vstst = rdiv(sql * vstl + sqr * vstr + (bystr - bystl) * signBx, dd);
+  R wstst = rdiv(sql * wstl + sqr * wstr + (bzstr - bzstl) * signBx, dd);
   R sqlr = rsqrt_(rhostl * rhostr);
-  R bystst = (sql * bystr + sqr * bystl + sqlr * (vstr - vstl) * signBx) / dd;
-  R bzstst = (sql * bzstr + sqr * bzstl + sqlr * (wstr - wstl) * signBx) / dd;
+  R bystst = rdiv(sql * bystr + sqr * bystl + sqlr * (vstr - vstl) * signBx, dd);
+  R bzstst = rdiv(sql * bzstr + sqr * bzstl + sqlr * (wstr - wstl) * signBx, dd);
   R vststdotbstst = sM * bx + vstst * bystst + wstst * bzstst;
   if (sM >= R(0)) {  // UL**
     R el = R(0.5) * pl[0] * (pl[1] * pl[1] + pl[2] * pl[2] + pl[3] * pl[3]) + cv * pl[4] +
            R(0.5) * (bx * bx + pl[6] * pl[6] + pl[7] * pl[7]);
     R vdotbl = pl[1] * bx + pl[2] * pl[6] + pl[3] * pl[7];
     R vstdotbstl = sM * bx + vstl * bystl + wstl * bzstl;
-    R estl = (slmul * el - pTL * pl[1] + pst * sM + bx * (vdotbl - vstdotbstl)) / slmsM;
+    R estl = rdiv(slmul * el - pTL * pl[1] + pst * sM + bx * (vdotbl - vstdotbstl), slmsM);
     R eststl = estl - sql * (vstdotbstl - vststdotbstst) * signBx;
     ff[0] = rhostl * sM;
     ff[1] = rhostl * (sM * sM) + pst - bx * bx;
@@ -339,7 +353,7 @@ __host__ __device__ inline void primfhlld(const Phys<R> &c, const R *pl, const R
            R(0.5) * (bx * bx + pr[6] * pr[6] + pr[7] * pr[7]);
     R vdotbr = pr[1] * bx + pr[2] * pr[6] + pr[3] * pr[7];
     R vstdotbstr = sM * bx + vstr * bystr + wstr * bzstr;
-    R estr = (srmur * er - pTR * pr[1] + pst * sM + bx * (vdotbr - vstdotbstr)) / srmsM;
+    R estr = rdiv(srmur * er - pTR * pr[1] + pst * sM + bx * (vdotbr - vstdotbstr), srmsM);
     R eststr = estr + sqr * (vstdotbstr - vststdotbstst) * signBx;
     ff[0] = rhostr * sM;
     ff[1] = rhostr * (sM * sM) + pst - bx * bx;
@@ -376,15 +390,15 @@ __host__ __device__ inline void primfhlld_sel(const Phys<R> &c, const R *pl, con
   R srmur = sr - pr[1];
   R rholul = pl[0] * pl[1];
   R rhorur = pr[0] * pr[1];
-  R sM = (srmur * rhorur - slmul * rholul - pTR + pTL) / (srmur * pr[0] - slmul * pl[0]);
+  R sM = rdiv(srmur * rhorur - slmul * rholul - pTR + pTL, srmur * pr[0] - slmul * pl[0]);
   R srmsM = sr - sM;
   R slmsM = sl - sM;
-  R rhostl = pl[0] * slmul / slmsM;
-  R rhostr = pr[0] * srmur / srmsM;
-  R sstl = sM - rabs(bx) / rsqrt_(rhostl);
-  R sstr = sM + rabs(bx) / rsqrt_(rhostr);
-  R pst = (srmur * pr[0] * pTL - slmul * pl[0] * pTR + pl[0] * pr[0] * srmur * slmul * (pr[1] - pl[1])) /
-          (srmur * pr[0] - slmul * pl[0]);
+  R rhostl = rdiv(pl[0] * slmul, slmsM);
+  R rhostr = rdiv(pr[0] * srmur, srmsM);
+  R sstl = sM - rdiv(rabs(bx), rsqrt_(rhostl));
+  R sstr = sM + rdiv(rabs(bx), rsqrt_(rhostr));
+  R pst = rdiv(srmur * pr[0] * pTL - slmul * pl[0] * pTR + pl[0] * pr[0] * srmur * slmul * (pr[1] - pl[1]),
+               srmur * pr[0] - slmul * pl[0]);
   // region, in the order the reference tests it
   const bool s1L = sstl >= R(0);
   const bool s1R = !s1L && sstr <= R(0);
@@ -412,16 +426,16 @@ __host__ __device__ inline void primfhlld_sel(const Phys<R> &c, const R *pl, con
          R(0.5) * (bx * bx + pK[6] * pK[6] + pK[7] * pK[7]);
   R vdotbK = pK[1] * bx + pK[2] * pK[6] + pK[3] * pK[7];
   R vstdotbstK = sM * bx + vstK * bystK + wstK * bzstK;
-  R estK = (sKmuK * eK - pTK * pK[1] + pst * sM + bx * (vdotbK - vstdotbstK)) / sKmsM;
+  R estK = rdiv(sKmuK * eK - pTK * pK[1] + pst * sM + bx * (vdotbK - vstdotbstK), sKmsM);
   R vq = vstK, wq = wstK, byq = bystK, bzq = bzstK, eq = estK, vdbq = vstdotbstK;
   if (!single) {
     R sql = rsqrt_(rhostl), sqr = rsqrt_(rhostr);
     R dd = sql + sqr;
-    R vstst = (sql * vstl + sqr * vstr + (bystr - bystl) * signBx) / dd;
-    R wstst = (sql * wstl + sqr * wstr + (bzstr - bzstl) * signBx) / dd;
+    R vstst = rdiv(sql * vstl + sqr * vstr + (bystr - bystl) * signBx, dd);
+    R wstst = rdiv(sql * wstl + sqr * wstr + (bzstr - bzstl) * signBx, dd);
     R sqlr = rsqrt_(rhostl * rhostr);
-    R bystst = (sql * bystr + sqr * bystl + sqlr * (vstr - vstl) * signBx) / dd;
-    R bzstst = (sql * bzstr + sqr * bzstl + sqlr * (wstr - wstl) * signBx) / dd;
+    R bystst = rdiv(sql * bystr + sqr * bystl + sqlr * (vstr - vstl) * signBx, dd);
+    R bzstst = rdiv(sql * bzstr + sqr * bzstl + sqlr * (wstr - wstl) * signBx, dd);
     R vststdotbstst = sM * bx + vstst * bystst + wstst * bzstst;
     R eststl = estK - sql * (vstdotbstK - vststdotbstst) * signBx;
     R eststr = estK + sqr * (vstdotbstK - vststdotbstst) * signBx;
