@@ -144,9 +144,43 @@ __device__ __forceinline__ void load_cell(const FusedArgs &a, const T *W, int b,
   }
 }
 
+// MHD primitives for the FP32 kernels (flow2prim, source/hydro_core.f90:
+// 323-378, with its floors): one reciprocal of the density for the
+// velocities; the pressure from the FP64 state in FP64 when fp64 (the
+// kinetic and magnetic energies cancel against E there), else in FP32.
+__device__ __forceinline__ void prims_mhd32(const DevCfg &c, const double *u, float *p, bool fp64) {
+  const float rho = (float)u[0];
+  const float ir = __builtin_amdgcn_rcpf(rho);
+  float pr;
+  if (fp64) {
+    double ke = 0.5 * (u[1] * u[1] + u[2] * u[2] + u[3] * u[3]) / u[0];
+    double me = 0.5 * (u[5] * u[5] + u[6] * u[6] + u[7] * u[7]);
+    pr = (float)((u[4] - ke - me) / c.pd.cv);
+  } else {
+    const float m1 = (float)u[1], m2 = (float)u[2], m3 = (float)u[3];
+    const float b1 = (float)u[5], b2 = (float)u[6], b3 = (float)u[7];
+    pr = ((float)u[4] - 0.5f * (m1 * m1 + m2 * m2 + m3 * m3) * ir - 0.5f * (b1 * b1 + b2 * b2 + b3 * b3)) *
+         __builtin_amdgcn_rcpf(c.pf.cv);
+  }
+  p[0] = rho < 1.0e-40f ? 1.0e-40f : rho;
+  p[1] = (float)u[1] * ir;
+  p[2] = (float)u[2] * ir;
+  p[3] = (float)u[3] * ir;
+  p[4] = pr < 1.0e-30f ? 1.0e-30f : pr;
+  p[5] = (float)u[5];
+  p[6] = (float)u[6];
+  p[7] = (float)u[7];
+}
+
 // primitives in the flux precision RF; fp64 = recover pressure in FP64
 template <int NV, typename RF>
 __device__ __forceinline__ void prims_r(const DevCfg &c, const double *u, RF *p, bool fp64) {
+  if constexpr (NV == 8 && sizeof(RF) == 4) {
+    if (c.pf.mhd) {
+      prims_mhd32(c, u, p, fp64);
+      return;
+    }
+  }
   if (fp64) {
     double pd[MAXV];
     flow2prim(c.pd, u, pd);
@@ -184,7 +218,7 @@ __device__ __forceinline__ float dt_cell_f(const DevCfg &c, const float *p, floa
     float cfx = cfast1(c.pf, p[4], p[0], p[5], p[6], p[7], p[5]);
     float cfy = cfast1(c.pf, p[4], p[0], p[5], p[6], p[7], p[6]);
     float cfz = cfast1(c.pf, p[4], p[0], p[5], p[6], p[7], p[7]);
-    r = fminf(fminf(dx / (fabsf(p[1]) + cfx), dy / (fabsf(p[2]) + cfy)), dz / (fabsf(p[3]) + cfz));
+    r = fminf(fminf(rdiv(dx, fabsf(p[1]) + cfx), rdiv(dy, fabsf(p[2]) + cfy)), rdiv(dz, fabsf(p[3]) + cfz));
   } else {
     float cs = sound(c.pf, p);
     r = fminf(fminf(dx / (fabsf(p[1]) + cs), dy / (fabsf(p[2]) + cs)), dz / (fabsf(p[3]) + cs));
@@ -205,9 +239,13 @@ __device__ __forceinline__ void finish_cell(const FusedArgs &a, int b, int i, in
 #pragma unroll
     for (int v = 0; v < NV; v++) Uo[v * NINT] = u[v];
     float uf[MAXV], pf[MAXV];
+    if constexpr (NV == 8) {
+      prims_r<NV, float>(c, u, pf, false);
+    } else {
 #pragma unroll
-    for (int v = 0; v < NV; v++) uf[v] = (float)u[v];
-    flow2prim(c.pf, uf, pf);
+      for (int v = 0; v < NV; v++) uf[v] = (float)u[v];
+      flow2prim(c.pf, uf, pf);
+    }
     dtmin = fmin(dtmin, (double)dt_cell_f<NV>(c, pf, (float)dx, (float)dy, (float)dz));
     return;
   }
