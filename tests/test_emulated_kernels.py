@@ -36,7 +36,7 @@ def _build():
              "-Wno-unknown-pragmas", "-I" + EMU, "-I" + os.path.join(ROOT, "walicxe3d_amd", "csrc"),
              "-I" + os.path.join(ROOT, "include")]
     objs = []
-    for src in ("wx_api", "wx_fused", "wx_plain"):
+    for src in ("wx_api", "wx_fused", "wx_fused_mhd", "wx_plain"):
         o = os.path.join(BUILD, src + ".o")
         cmd = [CXX, "-x", "c++"] + flags + ["-c", os.path.join(ROOT, "walicxe3d_amd", "csrc", src + ".hip"),
                                             "-o", o]

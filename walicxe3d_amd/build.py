@@ -32,6 +32,9 @@ UNITS = {
     # FP32 division and square root with the hardware's approximate lowering
     # (the FP64 arithmetic of the file is unaffected)
     "wx_fused.hip": ["-ffp-contract=off", "-fno-hip-fp32-correctly-rounded-divide-sqrt"],
+    # FP32 MHD stage kernels: FMA contraction on (same code path for every
+    # face, so shared faces still round identically)
+    "wx_fused_mhd.hip": ["-ffp-contract=fast", "-fno-hip-fp32-correctly-rounded-divide-sqrt"],
 }
 
 

@@ -37,10 +37,17 @@ __host__ __device__ inline float rcopysign(float a, float b) { return copysignf(
 // reciprocal (about 1 ulp) times the numerator, two instructions instead of
 // the range-scaled sequence; the FP32 form's fluxes carry FP32 rounding
 // anyway.  On the host (tests) FP32 division stays IEEE.
+__host__ __device__ inline float rcp32(float b) {
+#ifdef __HIP_DEVICE_COMPILE__
+  return __builtin_amdgcn_rcpf(b);
+#else
+  return 1.0f / b;
+#endif
+}
 __host__ __device__ inline double rdiv(double a, double b) { return a / b; }
 __host__ __device__ inline float rdiv(float a, float b) {
 #ifdef __HIP_DEVICE_COMPILE__
-  return a * __builtin_amdgcn_rcpf(b);
+  return a * rcp32(b);
 #else
   return a / b;
 #endif
