@@ -274,3 +274,33 @@ def test_fused_fp64_matches_oracle(oracle):
     uo, _ = o.get_state()
     assert np.array_equal(ug, uo)
     g.close()
+
+
+def test_fused_remesh_recaptures_graph():
+    """A new block list on a live context (what the host does after a regrid)
+    frees the fused state and recaptures the step graph: the run on the new
+    mesh equals, bit for bit, a fresh context on that mesh."""
+    from walicxe3d_amd import hip
+    w1 = W.get("winds4")
+    w2 = dataclasses.replace(w1, refine_radii=(0.3, 0.15, 0.08, 0.04))
+    b1 = np.asarray(w1.blocks(), dtype=np.int32)
+    b2 = np.asarray(w2.blocks(), dtype=np.int32)
+    assert len(b2) != len(b1)
+    say("case remesh winds4:", len(b1), "->", len(b2), "blocks")
+    g = gpu(w1, hip.FUSED_FP32, b1)
+    g.upload(ic_in_order(w1, b1))
+    g.run(2)
+    g.set_mesh(b2)
+    U2 = ic_in_order(w2, b2)
+    g.upload(U2)
+    g.it.value, g.dt.value, g.time.value, g.nextout.value = 0, 0.0, 0.0, 1
+    g.run(3)
+    ua = g.download(prim=False)
+    h = gpu(w2, hip.FUSED_FP32, b2)
+    h.upload(U2)
+    h.run(3)
+    ub = h.download(prim=False)
+    assert g.time.value == h.time.value
+    assert np.array_equal(ua, ub)
+    g.close()
+    h.close()

@@ -245,14 +245,20 @@ __device__ __forceinline__ void finish_cell(const FusedArgs &a, int b, int i, in
 // Stage kernel.  S2 = second stage; VISC = artificial viscosity active;
 // RF = flux/reconstruction arithmetic; ST = storage type of UP; SOLVER =
 // Riemann solver; LIM = limiter fixed at compile time, or LIM_RUNTIME.
-// Launch bound: 256 threads, at least three waves per SIMD.
+// Launch bound: 256 threads, at least three waves per SIMD (four for the MHD
+// first stage).
 // ---------------------------------------------------------------------------
 constexpr int LIM_RUNTIME = -100;
 #ifndef WX_STAGE_MIN_WAVES
 #define WX_STAGE_MIN_WAVES 3
 #endif
+// the MHD first stage (no limiter, fewer live values) runs best at four
+#ifndef WX_MHD_STAGE1_MIN_WAVES
+#define WX_MHD_STAGE1_MIN_WAVES 4
+#endif
 template <int NV, bool S2, bool VISC, typename RF, typename ST, int SOLVER, int LIM>
-__global__ void __launch_bounds__(256, WX_STAGE_MIN_WAVES) k_stage(FusedArgs a) {
+__global__ void __launch_bounds__(256, (NV == 8 && !S2) ? WX_MHD_STAGE1_MIN_WAVES : WX_STAGE_MIN_WAVES)
+    k_stage(FusedArgs a) {
   constexpr int ORD = S2 ? 2 : 1;
   constexpr int H = ORD;
   constexpr int W = NC + 2 * H;

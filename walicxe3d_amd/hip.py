@@ -145,17 +145,23 @@ class Walicxe3D:
         h = C.c_void_p()
         _check(L.wx_init(C.byref(self.cfg), C.byref(h)), "wx_init")
         self._h = h
-        m = w.mesh()
-        self.bids = np.ascontiguousarray(bids if bids is not None else w.blocks(), dtype=np.int32)
-        dx = np.ascontiguousarray(np.array(m.dx, dtype=np.float64).reshape(-1))
-        nroot = (C.c_int * 3)(*m.nroot)
-        _check(L.wx_set_mesh(h, nroot, m.maxlev, _p(dx), len(self.bids), self.bids.ctypes.data_as(_ip)),
-               "wx_set_mesh")
-        self.nblk = len(self.bids)
+        self.set_mesh(bids if bids is not None else w.blocks())
         self.it = C.c_int(0)
         self.dt = C.c_double(0.0)
         self.time = C.c_double(0.0)
         self.nextout = C.c_int(1)
+
+    def set_mesh(self, bids):
+        """(Re)send the active block list (after a regrid): rebuilds the face
+        table and, in the fused form, recaptures the step graph on the next
+        run.  The state must be uploaded again afterwards."""
+        m = self.w.mesh()
+        self.bids = np.ascontiguousarray(bids, dtype=np.int32)
+        dx = np.ascontiguousarray(np.array(m.dx, dtype=np.float64).reshape(-1))
+        nroot = (C.c_int * 3)(*m.nroot)
+        _check(load().wx_set_mesh(self._h, nroot, m.maxlev, _p(dx), len(self.bids),
+                                  self.bids.ctypes.data_as(_ip)), "wx_set_mesh")
+        self.nblk = len(self.bids)
 
     def close(self):
         if getattr(self, "_h", None):
