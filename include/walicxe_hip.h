@@ -137,6 +137,18 @@ int wx_fluxes_update(void *ctx, int order, double dtp);
    which = 0 U, 1 UP, 2 PRIM.  Plain form only. */
 int wx_copy_raw(void *ctx, int which, double *out);
 
+/* Change of the active block list with the state kept on the device
+   (admesh's topology change, source/amr.f90:44-133, with the data motion of
+   refineBlock, :284-384, and coarsenBlock, :1430-1650): a block present in
+   both lists is copied, a new block under an old ancestor takes the
+   ancestor's cells by injection, a new block whose 8 children were active
+   takes their 2x2x2 means (z outer, x inner, then /8).  Then as wx_set_mesh
+   followed by an upload of that state (face table rebuilt, PRIM or the fused
+   state recomputed, the step graph recaptured on the next wx_run).  Returns
+   WX_ETOPO, with the mesh and state untouched, when a new block would need a
+   restriction over more than one level. */
+int wx_remesh(void *ctx, int nblk, const int *bids);
+
 /* Tile/launch choice of the fused form (autotuner): kind 0 = stage kernel
    block size, value as documented in the implementation. */
 int wx_set_tuning(void *ctx, int kind, int value);

@@ -29,7 +29,15 @@ static int run_case(const char *path, int nsteps) {
   std::vector<int> bids(nblk);
   fread(tT.data(), 8, ntab, f); fread(tC.data(), 8, ntab, f); fread(wnd.data(), 8, 8 * nwind, f);
   fread(dx.data(), 8, 3 * maxlev, f); fread(bids.data(), 4, nblk, f); fread(U.data(), 8, U.size(), f);
-  double dtout_tsc[2]; fread(dtout_tsc, 8, 2, f); fclose(f);
+  double dtout_tsc[2]; fread(dtout_tsc, 8, 2, f);
+  // optional: a second block list to move the state to (wx_remesh) after the run
+  int nb2 = 0;
+  std::vector<int> bids2;
+  if (fread(&nb2, 4, 1, f) == 1 && nb2 > 0) {
+    bids2.resize(nb2);
+    fread(bids2.data(), 4, nb2, f);
+  }
+  fclose(f);
   cfg.tab_logT = ntab ? tT.data() : nullptr; cfg.tab_loglam = ntab ? tC.data() : nullptr;
   cfg.winds = nwind ? wnd.data() : nullptr;
   void *ctx;
@@ -44,6 +52,15 @@ static int run_case(const char *path, int nsteps) {
     rc = wx_run(ctx, nsteps, &it, &dt, &time, &nextout, dtout_tsc[0], dtout_tsc[1]);
   }
   if (rc < 0) { printf("run %d %s\n", rc, wx_last_error()); return 1; }
+  if (nb2 > 0) {
+    rc = wx_remesh(ctx, nb2, bids2.data());
+    if (rc < 0) { printf("remesh %d %s\n", rc, wx_last_error()); return 1; }
+    rc = wx_remesh(ctx, nblk, bids.data());
+    if (rc < 0) { printf("remesh back %d %s\n", rc, wx_last_error()); return 1; }
+    if (cfg.form == WX_FORM_PLAIN_FP64) rc = wx_step(ctx, &it, &dt, &time, &nextout, dtout_tsc[0], dtout_tsc[1]);
+    else rc = wx_run(ctx, 1, &it, &dt, &time, &nextout, dtout_tsc[0], dtout_tsc[1]);
+    if (rc < 0) { printf("run after remesh %d %s\n", rc, wx_last_error()); return 1; }
+  }
   std::vector<double> Uo(U.size());
   wx_download_state(ctx, Uo.data(), nullptr, WX_LAYOUT_BLOCKS, 0, nullptr);
   { std::string fn = std::string(path) + ".out"; FILE *fo = fopen(fn.c_str(), "wb"); fwrite(Uo.data(), 8, Uo.size(), fo); fclose(fo); }

@@ -49,7 +49,7 @@ def _build():
     return exe
 
 
-def _write_input(path, w, form):
+def _write_input(path, w, form, remesh_to=None):
     slots, blocks, U0 = W.initial_state(w)
     cfg = hip.make_config(w, form)
     raw = bytes(cfg)
@@ -65,6 +65,9 @@ def _write_input(path, w, form):
         f.write(np.asarray(blocks, dtype="<i4").tobytes())
         f.write(np.ascontiguousarray(U0, dtype="<f8").tobytes())
         f.write(struct.pack("<2d", w.dtout, w.t_sc))
+        if remesh_to is not None:
+            f.write(struct.pack("<i", len(remesh_to)))
+            f.write(np.asarray(remesh_to, dtype="<i4").tobytes())
     return blocks, U0
 
 
@@ -83,13 +86,21 @@ def _run(emu, args, tmp):
 
 
 def test_every_kernel_in_bounds(emu, tmp_path):
+    import dataclasses
     args = []
     for n, (name, form, steps) in enumerate(CASES):
         p = str(tmp_path / ("in%d.bin" % n))
         _write_input(p, W.get(name), form)
         args += [p, str(steps)]
+    # the device remesh (refine, then coarsen back) in both forms
+    w = W.get("winds4")
+    finer = [int(b) for b in dataclasses.replace(w, refine_radii=(0.3, 0.15, 0.08, 0.04)).blocks()]
+    for n, form in enumerate((hip.FUSED_FP32, hip.PLAIN_FP64)):
+        p = str(tmp_path / ("remesh%d.bin" % n))
+        _write_input(p, w, form, remesh_to=finer)
+        args += [p, "1"]
     out = _run(emu, args, str(tmp_path))
-    assert out.count("it ") == len(CASES), out
+    assert out.count("it ") == len(CASES) + 2, out
 
 
 def test_emulation_reproduces_fused_fp64(emu, tmp_path, oracle):

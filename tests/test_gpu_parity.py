@@ -304,3 +304,41 @@ def test_fused_remesh_recaptures_graph():
     assert np.array_equal(ua, ub)
     g.close()
     h.close()
+
+
+@pytest.mark.parametrize("form", ["fused_fp32", "plain"])
+def test_device_remesh_matches_host_regrid(form):
+    """wx_remesh carries the state to a new block list on the device (copy,
+    octant injection, 2x2x2 restriction); it equals the host regrid of the
+    downloaded state bit for bit, in both directions, and the next steps
+    equal those of a context given that state by upload."""
+    from walicxe3d_amd import hip
+    from walicxe3d_amd import regrid
+    f = hip.FUSED_FP32 if form == "fused_fp32" else hip.PLAIN_FP64
+    w1 = W.get("winds4")
+    w2 = dataclasses.replace(w1, refine_radii=(0.3, 0.15, 0.08, 0.04))
+    m = w1.mesh()
+    b1 = np.asarray(w1.blocks(), dtype=np.int32)
+    b2 = np.asarray(w2.blocks(), dtype=np.int32)
+    say("case device remesh", form, len(b1), "<->", len(b2))
+    g = gpu(w1, f, b1)
+    g.upload(ic_in_order(w1, b1))
+    for _ in range(2):
+        g.step()
+    u1 = g.download(prim=False)
+    g.remesh(b2)                               # refine: copies and injections
+    u2 = g.download(prim=False)
+    assert np.array_equal(u2, regrid.remap(m, b1, u1, b2))
+    g.remesh(b1)                               # coarsen back: copies and restrictions
+    u3 = g.download(prim=False)
+    assert np.array_equal(u3, regrid.remap(m, b2, u2, b1))
+    g.remesh(b2)
+    h = gpu(w2, f, b2)
+    h.upload(regrid.remap(m, b1, u3, b2))
+    h.it.value, h.dt.value, h.time.value, h.nextout.value = g.it.value, g.dt.value, g.time.value, g.nextout.value
+    for _ in range(2):
+        g.step()
+        h.step()
+    assert np.array_equal(g.download(prim=False), h.download(prim=False))
+    g.close()
+    h.close()

@@ -40,6 +40,7 @@ struct Ctx {
   hipStream_t st = nullptr;
   // mesh
   int nblk = 0, maxlev = 0, nroot[3] = {0, 0, 0};
+  std::vector<double> dxyz;          // grid spacings per level (wx_set_mesh)
   std::vector<int> bids;
   std::vector<BlockInfo> info;
   std::unordered_map<int, int> index;
@@ -325,6 +326,104 @@ int need_state(Ctx *c) {
 
 }  // namespace
 
+
+// ---------------------------------------------------------------------------
+// Remesh on the device: carries the interior state from the current block
+// list to a new one, with the data motion of the reference's refineBlock
+// (source/amr.f90:284-384: the father's octant copied into each child) and
+// coarsenBlock (:1430-1650: each father cell the mean of its 2x2x2 children
+// cells), without a round trip through host memory.
+// ---------------------------------------------------------------------------
+struct RemapSrc {
+  int kind;          // 0 copy, 1 injection from an ancestor, 2 mean of 8 children
+  int src;           // old block index (copy, injection)
+  int shift;         // levels between the new block and its ancestor
+  int f0[3];         // (coords-1)*16 of the new block at its level
+  int a0[3];         // (coords-1)*16 of the ancestor at its level
+  int kids[8];       // old indices of the children, x fastest then y, z
+};
+
+__global__ void __launch_bounds__(256) k_remap(int nv, const RemapSrc *map, const double *src, double *dst) {
+  const int b = blockIdx.x, v = blockIdx.y;
+  const int i = threadIdx.x & 15, j = threadIdx.x >> 4;
+  const RemapSrc m = map[b];
+  double *D = dst + ((size_t)b * nv + v) * NINT;
+  for (int k = 0; k < NC; k++) {
+    const int q = (k * NC + j) * NC + i;
+    double val;
+    if (m.kind == 0) {
+      val = src[((size_t)m.src * nv + v) * NINT + q];
+    } else if (m.kind == 1) {
+      const int ai = ((m.f0[0] + i) >> m.shift) - m.a0[0];
+      const int aj = ((m.f0[1] + j) >> m.shift) - m.a0[1];
+      const int ak = ((m.f0[2] + k) >> m.shift) - m.a0[2];
+      val = src[((size_t)m.src * nv + v) * NINT + (ak * NC + aj) * NC + ai];
+    } else {
+      const int c = (i >> 3) + 2 * (j >> 3) + 4 * (k >> 3);
+      const double *S = src + ((size_t)m.kids[c] * nv + v) * NINT;
+      const int i0 = 2 * (i & 7), j0 = 2 * (j & 7), k0 = 2 * (k & 7);
+      // z outermost, x innermost
+      double acc = 0.0;
+      for (int dk = 0; dk < 2; dk++)
+        for (int dj = 0; dj < 2; dj++)
+          for (int di = 0; di < 2; di++) acc = acc + S[((k0 + dk) * NC + (j0 + dj)) * NC + (i0 + di)];
+      val = acc / 8.0;
+    }
+    D[q] = val;
+  }
+}
+
+// Source of every new block; fails when a new block has neither an old
+// copy, an old ancestor, nor all 8 children in the old list.
+static int remap_table(Ctx *c, int nblk, const int *bids, std::vector<RemapSrc> &out) {
+  Numbering N;
+  N.init(c->nroot, c->maxlev);
+  out.assign(nblk, RemapSrc{});
+  for (int n = 0; n < nblk; n++) {
+    RemapSrc &m = out[n];
+    const long b = bids[n];
+    const int lev = N.level(b);
+    if (lev < 0) return fail(WX_ETOPO, "block ID outside the mesh: " + std::to_string(b));
+    auto it = c->index.find((int)b);
+    if (it != c->index.end()) {
+      m.kind = 0;
+      m.src = it->second;
+      continue;
+    }
+    int bc[3];
+    N.coords(b, lev, bc);
+    // nearest old ancestor
+    int p[3] = {bc[0], bc[1], bc[2]};
+    bool found = false;
+    for (int l = lev - 1, d = 1; l >= 1; l--, d++) {
+      for (int a = 0; a < 3; a++) p[a] = (p[a] + 1) / 2;
+      auto ia = c->index.find((int)N.bid(l, p));
+      if (ia != c->index.end()) {
+        m.kind = 1;
+        m.src = ia->second;
+        m.shift = d;
+        for (int a = 0; a < 3; a++) {
+          m.f0[a] = (bc[a] - 1) * NC;
+          m.a0[a] = (p[a] - 1) * NC;
+        }
+        found = true;
+        break;
+      }
+    }
+    if (found) continue;
+    if (lev >= c->maxlev) return fail(WX_ETOPO, "no source for block " + std::to_string(b));
+    m.kind = 2;
+    for (int q = 0; q < 8; q++) {
+      int ch[3] = {2 * bc[0] - 1 + (q & 1), 2 * bc[1] - 1 + ((q >> 1) & 1), 2 * bc[2] - 1 + ((q >> 2) & 1)};
+      auto ic = c->index.find((int)N.bid(lev + 1, ch));
+      if (ic == c->index.end())
+        return fail(WX_ETOPO, "block " + std::to_string(b) + " needs a restriction over more than one level");
+      m.kids[q] = ic->second;
+    }
+  }
+  return WX_OK;
+}
+
 extern "C" {
 
 const char *wx_last_error(void) { return g_err.c_str(); }
@@ -404,6 +503,7 @@ int wx_set_mesh(void *ctx, const int nroot[3], int maxlev, const double *dxyz, i
   c->nblk = nblk;
   c->maxlev = maxlev;
   for (int d = 0; d < 3; d++) c->nroot[d] = nroot[d];
+  if (dxyz != c->dxyz.data()) c->dxyz.assign(dxyz, dxyz + 3 * maxlev);
   int rc = build_topology(c, bids);
   if (rc != WX_OK) return rc;
   size_t n = (size_t)nblk * c->cfg.nv * NCELL;
@@ -426,6 +526,24 @@ int wx_set_mesh(void *ctx, const int nroot[3], int maxlev, const double *dxyz, i
   return WX_OK;
 }
 
+// Interior state [nblk][nv][16^3] already on the device into the context:
+// ghosted U with zero ghosts, then PRIM (plain form) or the fused state and
+// its first dt candidate.
+static int upload_device(Ctx *c, const double *d_in) {
+  int nv = c->cfg.nv;
+  size_t total = (size_t)c->nblk * nv * NCELL;
+  hipLaunchKernelGGL(k_scatter, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, c->st, nv, c->nblk, d_in, c->dU, 1);
+  int rc = launch_check("upload");
+  HIPCHK(hipStreamSynchronize(c->st));
+  if (rc != WX_OK) return rc;
+  c->have_state = true;
+  if (is_fused(c)) return fused_upload(c->fused, c->dc, c->dU, c->st) ? WX_EHIP : WX_OK;
+  long bad = 0;
+  rc = update_prims(c, &bad);
+  if (rc != WX_OK) return rc;
+  return (int)bad;
+}
+
 int wx_upload_state(void *ctx, const double *u, int layout, int nbmax, const int *lb) {
   Ctx *c = as_ctx(ctx);
   if (!c || !u) return fail(WX_EINVAL, "null argument");
@@ -445,18 +563,9 @@ int wx_upload_state(void *ctx, const double *u, int layout, int nbmax, const int
   double *d_in = nullptr;
   HIPCHK(hipMalloc(&d_in, ni * sizeof(double)));
   HIPCHK(hipMemcpyAsync(d_in, src, ni * sizeof(double), hipMemcpyHostToDevice, c->st));
-  size_t total = (size_t)c->nblk * nv * NCELL;
-  hipLaunchKernelGGL(k_scatter, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, c->st, nv, c->nblk, d_in, c->dU, 1);
-  int rc = launch_check("upload");
-  HIPCHK(hipStreamSynchronize(c->st));
+  int rc = upload_device(c, d_in);
   hipFree(d_in);
-  if (rc != WX_OK) return rc;
-  c->have_state = true;
-  if (is_fused(c)) return fused_upload(c->fused, c->dc, c->dU, c->st) ? WX_EHIP : WX_OK;
-  long bad = 0;
-  rc = update_prims(c, &bad);
-  if (rc != WX_OK) return rc;
-  return (int)bad;
+  return rc;
 }
 
 int wx_upload_prims(void *ctx, const double *p, int layout, int nbmax, const int *lb) {
@@ -673,6 +782,41 @@ int wx_last_timing(void *ctx, double *s1, double *s2, double *tot) {
   if (!c) return fail(WX_EINVAL, "null context");
   fused_last_timing(c->fused, s1, s2, tot);
   return WX_OK;
+}
+
+int wx_remesh(void *ctx, int nblk, const int *bids) {
+  Ctx *c = as_ctx(ctx);
+  int rc = need_state(c);
+  if (rc != WX_OK) return rc;
+  if (!bids || nblk < 1) return fail(WX_EINVAL, "bad block list");
+  std::vector<RemapSrc> table;
+  rc = remap_table(c, nblk, bids, table);
+  if (rc != WX_OK) return rc;              // the current mesh and state are untouched
+  const int nv = c->cfg.nv;
+  HIPCHK(hipStreamSynchronize(c->st));
+  if (is_fused(c) && fused_download(c->fused, c->dc, c->dU, c->dP, c->st))
+    return fail(WX_EHIP, "fused state read-back failed");
+  size_t nold = (size_t)c->nblk * nv * NINT, nnew = (size_t)nblk * nv * NINT;
+  double *d_old = nullptr, *d_new = nullptr;
+  RemapSrc *d_map = nullptr;
+  HIPCHK(hipMalloc(&d_old, nold * sizeof(double)));
+  HIPCHK(hipMalloc(&d_new, nnew * sizeof(double)));
+  HIPCHK(hipMalloc(&d_map, table.size() * sizeof(RemapSrc)));
+  hipLaunchKernelGGL(k_gather, dim3((unsigned)((nold + 255) / 256)), dim3(256), 0, c->st, nv, c->nblk, c->dU, d_old);
+  HIPCHK(hipMemcpyAsync(d_map, table.data(), table.size() * sizeof(RemapSrc), hipMemcpyHostToDevice, c->st));
+  hipLaunchKernelGGL(k_remap, dim3(nblk, nv), dim3(256), 0, c->st, nv, d_map, d_old, d_new);
+  rc = launch_check("remesh");
+  HIPCHK(hipStreamSynchronize(c->st));
+  hipFree(d_old);
+  hipFree(d_map);
+  if (rc == WX_OK) {
+    const std::vector<double> dxyz = c->dxyz;
+    int nroot[3] = {c->nroot[0], c->nroot[1], c->nroot[2]};
+    rc = wx_set_mesh(ctx, nroot, c->maxlev, dxyz.data(), nblk, bids);
+  }
+  if (rc == WX_OK) rc = upload_device(c, d_new);
+  hipFree(d_new);
+  return rc;
 }
 
 }  // extern "C"

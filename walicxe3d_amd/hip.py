@@ -65,6 +65,7 @@ SIGNATURES = {
     "wx_set_tuning": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "wx_last_timing": (C.c_int, [C.c_void_p, _dp, _dp, _dp]),
     "wx_time_stages": (C.c_int, [C.c_void_p, C.c_int]),
+    "wx_remesh": (C.c_int, [C.c_void_p, C.c_int, _ip]),
 }
 
 _lib = None
@@ -162,6 +163,15 @@ class Walicxe3D:
         _check(load().wx_set_mesh(self._h, nroot, m.maxlev, _p(dx), len(self.bids),
                                   self.bids.ctypes.data_as(_ip)), "wx_set_mesh")
         self.nblk = len(self.bids)
+
+    def remesh(self, bids):
+        """New active block list with the state carried over on the device
+        (copy / octant injection / 2x2x2 restriction)."""
+        b = np.ascontiguousarray(bids, dtype=np.int32)
+        rc = _check(load().wx_remesh(self._h, len(b), b.ctypes.data_as(_ip)), "wx_remesh")
+        self.bids = b
+        self.nblk = len(b)
+        return rc
 
     def close(self):
         if getattr(self, "_h", None):
