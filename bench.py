@@ -104,6 +104,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="skip the launch autotuner (profiling runs)")
+    ap.add_argument("--remesh-every", type=int, default=0,
+                    help="AMR workloads: change the block list every N steps (device remesh, graph recapture), "
+                         "alternating with a 6%% wider refinement")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the block list of this path lives on a single device "
@@ -134,10 +137,42 @@ def main():
         log("autotune: %s" % ", ".join("grid %d xcd %d %.3f ms" % (r["grid"], r["xcd_map"], r["ms"])
                                        for r in table))
     f.run(args.warmup)
-    t0 = time.perf_counter()
-    f.run(args.steps)                      # returns after a stream synchronize
-    t_fused = time.perf_counter() - t0
-    fused_zps = zones * args.steps / t_fused
+    remesh = None
+    if args.remesh_every > 0:
+        import dataclasses
+        wB = dataclasses.replace(w, refine_radii=tuple(r * 1.06 for r in w.refine_radii))
+        meshes = [bids, np.asarray(wB.blocks(), dtype=np.int32)]
+        # one untimed round trip so both meshes' allocations and graphs have been built once
+        f.remesh(meshes[1])
+        f.run(1)
+        f.remesh(meshes[0])
+        f.run(1)
+        cur, done, zsum, t_rm, n_rm = 0, 0, 0, 0.0, 0
+        t0 = time.perf_counter()
+        while done < args.steps:
+            n = min(args.remesh_every, args.steps - done)
+            f.run(n)
+            zsum += len(meshes[cur]) * 4096 * n
+            done += n
+            if done < args.steps:
+                tr = time.perf_counter()
+                cur = 1 - cur
+                f.remesh(meshes[cur])
+                t_rm += time.perf_counter() - tr
+                n_rm += 1
+        t_fused = time.perf_counter() - t0
+        fused_zps = zsum / t_fused
+        remesh = {"every": args.remesh_every, "count": n_rm, "ms_per_remesh": 1e3 * t_rm / max(n_rm, 1),
+                  "blocks": [len(m_) for m_ in meshes],
+                  "zone_updates_per_s_without_remesh": zsum / max(t_fused - t_rm, 1e-12)}
+        log("remesh every %d steps: %d remeshes, %.1f ms each" % (args.remesh_every, n_rm, remesh["ms_per_remesh"]))
+        if cur != 0:
+            f.remesh(meshes[0])
+    else:
+        t0 = time.perf_counter()
+        f.run(args.steps)                      # returns after a stream synchronize
+        t_fused = time.perf_counter() - t0
+        fused_zps = zones * args.steps / t_fused
     s1, s2 = f.time_stages(3)
     log("fused: %.3f ms/step, %.3e zone-updates/s; stage1 %.3f ms, stage2 %.3f ms"
         % (1e3 * t_fused / args.steps, fused_zps, s1, s2))
@@ -211,7 +246,7 @@ def main():
         "config": {"workload": "%s: %s" % (w.name, w.description), "blocks": nblk, "zones": zones,
                    "form": "fused FP32 bulk, FP64 conserved accumulation", "solver": w.solver,
                    "limiter": w.limiter, "step_hbm_roofline_frac": (fused_zps * step_bytes / zones) / HBM_PEAK,
-                   "tuning": {"grid": best["grid"], "xcd_map": best["xcd_map"]}},
+                   "tuning": {"grid": best["grid"], "xcd_map": best["xcd_map"]}, "remesh": remesh},
         "roofline": roof,
         "cpu_baseline": cpu,
         "plain_fp64": plain,
