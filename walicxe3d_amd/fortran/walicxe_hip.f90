@@ -61,6 +61,13 @@ module walicxe_hip
       integer(c_int), intent(in) :: bids(*)
     end function
 
+    integer(c_int) function wx_remesh(ctx, nblk, bids) bind(C, name="wx_remesh")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nblk
+      integer(c_int), intent(in) :: bids(*)
+    end function
+
     integer(c_int) function wx_upload_state(ctx, u, layout, nbmax, local_blocks) bind(C, name="wx_upload_state")
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx, u, local_blocks
