@@ -143,11 +143,55 @@ int wx_copy_raw(void *ctx, int which, double *out);
    both lists is copied, a new block under an old ancestor takes the
    ancestor's cells by injection, a new block whose 8 children were active
    takes their 2x2x2 means (z outer, x inner, then /8).  Then as wx_set_mesh
-   followed by an upload of that state (face table rebuilt, PRIM or the fused
-   state recomputed, the step graph recaptured on the next wx_run).  Returns
-   WX_ETOPO, with the mesh and state untouched, when a new block would need a
-   restriction over more than one level. */
+   followed by an upload of that state (face table rebuilt, the step graph
+   recaptured on the next wx_run).  The plain form carries PRIM as the
+   reference does: kept by copied blocks, injected into refined children,
+   recomputed from U in new coarse blocks; the fused form recomputes its
+   state from U.  Returns WX_ETOPO, with the mesh and state untouched, when a
+   new block would need a restriction over more than one level. */
 int wx_remesh(void *ctx, int nblk, const int *bids);
+
+/* Refinement flags (WX_FLAG_*) of admesh */
+#define WX_FLAG_NONE 0
+#define WX_FLAG_REFINE 1
+#define WX_FLAG_COARSE 2
+
+/* markByPhysical (source/amr.f90:142-230) of every block, on the device:
+   the largest relative gradient |q(i+1)-q(i-1)|/q(i)/(2 dx) of pressure and
+   density over the interior (maxgrad, may be NULL) and the flag it gives:
+   REFINE at or above refine_thres (NONE at the maximum level), else COARSE
+   at or below coarse_thres above level 1, else NONE.  The plain form reads
+   PRIM with the ghost layer its last step left (the exchange of the state the
+   step started from), as the reference's admesh does after a step.  The
+   fused form keeps no ghost layers: it reads the layer wx_record_ghosts
+   recorded, or else exchanges the current state. */
+int wx_mark_blocks(void *ctx, double refine_thres, double coarse_thres, int *flags, double *maxgrad);
+
+/* Fused form: records the depth-1 PRIM ghost layer of the current state (the
+   boundary + calcPrimsAll(CELLS_GHOST) of the reference's first stage,
+   source/godunov.f90:78-83), which wx_mark_blocks / wx_admesh then read, so
+   that called right before the step that admesh follows the flags are the
+   reference's (its markByPhysical reads that layer).  Kept until the next
+   upload or mesh change.  No-op in the plain form. */
+int wx_record_ghosts(void *ctx);
+
+/* The mesh change of admesh (source/amr.f90:44-133) for given flags, host
+   only: checkProximity (:1109-1254), then every block flagged REFINE
+   replaced by its 8 children (refineBlock, :284-384) and every family of 8
+   siblings all flagged COARSE by their father (coarseBlocks, :1430-1650).
+   The new list keeps the old order (children in place of their father, x
+   fastest; a father in place of the first of its children).  flags_out (may
+   be NULL) gets the flags after the proximity rule.  *nblk_new is set even
+   when the list does not fit in cap (WX_ENOMEM). */
+int wx_admesh_plan(const int nroot[3], int maxlev, int nblk, const int *bids, const int *flags,
+                   int *flags_out, int cap, int *nblk_new, int *bids_new);
+
+/* admesh (source/amr.f90:44-133) on the device state: wx_mark_blocks,
+   wx_admesh_plan, then wx_remesh to the new list (skipped when the list is
+   unchanged).  The new list is returned in bids_new (cap entries).  Returns
+   WX_ENOMEM or WX_ETOPO (a list beyond cap, or a mesh the face table cannot
+   hold, such as a jump of two levels) with the mesh and state untouched. */
+int wx_admesh(void *ctx, double refine_thres, double coarse_thres, int cap, int *nblk_new, int *bids_new);
 
 /* Tile/launch choice of the fused form (autotuner): kind 0 = stage kernel
    block size, value as documented in the implementation. */

@@ -66,8 +66,16 @@ def flit(x):
     return s
 
 
-def parameters_text(w, nprocs=1):
-    nblk_max = w.max_blocks_ref()
+def amr_blocks(w):
+    """Block slots of a build whose mesh adapts every step: room for every
+    block of the finest level (capped), as refineBlock aborts when the slots
+    run out (source/amr.f90:318-324)."""
+    finest = w.nroot[0] * w.nroot[1] * w.nroot[2] * 8 ** (w.maxlev - 1)
+    return min(finest + 64, 6000)
+
+
+def parameters_text(w, nprocs=1, amr=False):
+    nblk_max = amr_blocks(w) if amr else w.max_blocks_ref()
     nv = (8 if w.mhd else 5) + 1
     block_ram = 20 ** 3 * nv * 8
     # RAM_per_proc such that nbMaxProc (source/parameters.f90:342-343) >= needed
@@ -120,12 +128,13 @@ def gpu_sources():
     return out
 
 
-def build(name, gpu=False, nprocs=1):
+def build(name, gpu=False, nprocs=1, amr=False):
     w = workloads.get(name)
-    bdir = os.path.join(OUT, name + ("_gpu" if gpu else "") + ("_np%d" % nprocs if nprocs > 1 else ""))
+    bdir = os.path.join(OUT, name + ("_gpu" if gpu else "") + ("_np%d" % nprocs if nprocs > 1 else "")
+                        + ("_amr" if amr else ""))
     os.makedirs(bdir, exist_ok=True)
     with open(os.path.join(bdir, "parameters.f90"), "w") as f:
-        f.write(parameters_text(w, nprocs))
+        f.write(parameters_text(w, nprocs, amr))
     for fn in ("user.f90", "harness.f90"):
         shutil.copyfile(os.path.join(HERE, "ref", fn), os.path.join(bdir, fn))
     flags = ["-cpp", "-DMPIP", "-DDOUBLEP", "-fdefault-real-8", os.environ.get("WX_FOPT", "-O2"),
@@ -166,23 +175,29 @@ def main(argv):
     gpu = "--gpu" in argv
     argv = [a for a in argv if not a.startswith("--")]
     names = argv or workloads.oracle_builds()
-    jobs = [(n, False, 1) for n in names]
+    jobs = [(n, False, 1, False) for n in names]
     if gpu or not argv:
         # drop-in builds used by tests/test_gpu_fortran.py
-        jobs += [(n, True, 1) for n in (argv or DROPIN_WORKLOADS)]
+        jobs += [(n, True, 1, False) for n in (argv or DROPIN_WORKLOADS)]
     if not argv or "--baseline" in sys.argv:
         # multi-rank builds for the CPU baseline of bench.py (mpiexec -n N)
-        jobs += [(n, False, np_) for n, np_ in BASELINE_BUILDS]
+        jobs += [(n, False, np_, False) for n, np_ in BASELINE_BUILDS]
+    if not argv or "--amr" in sys.argv:
+        # adaptive-mesh builds (admesh every step) for the AMR fixtures
+        jobs += [(n, False, 1, True) for n in (argv or AMR_WORKLOADS)]
     from concurrent.futures import ThreadPoolExecutor
     with ThreadPoolExecutor(max_workers=4) as ex:
-        for (n, g, np_), exe in zip(jobs, ex.map(lambda j: build(*j), jobs)):
-            print("building", n, "(gpu drop-in)" if g else "", "(%d ranks)" % np_ if np_ > 1 else "", "->", exe)
+        for (n, g, np_, amr), exe in zip(jobs, ex.map(lambda j: build(*j), jobs)):
+            print("building", n, "(gpu drop-in)" if g else "", "(%d ranks)" % np_ if np_ > 1 else "",
+                  "(adaptive mesh)" if amr else "", "->", exe)
     return 0
 
 
 DROPIN_WORKLOADS = ["sedov", "ot64", "amr_small", "blast_reflect"]
 # (workload, MPI ranks): 16 ranks = the host cores one GPU of the box comes with
 BASELINE_BUILDS = [("ot64", 16)]
+# workloads whose fixtures include the reference's mesh adaptation
+AMR_WORKLOADS = ["amr_adapt"]
 
 
 if __name__ == "__main__":

@@ -13,6 +13,13 @@
 !                     after writing an output.
 !   mode   = 'unit' : evaluates one reference routine on every record of
 !                     unit_in.bin and writes unit_out.bin (op codes below).
+!   amr    = .true. : after cooling, every step also runs the reference's
+!                     mesh update as main.f90:103-106 does (admesh, then
+!                     loadBalance), writing first the markByPhysical flag of
+!                     every active block (flags_NNNNN.bin: nact, then bID and
+!                     flag pairs) and afterwards the active bIDs in slot order
+!                     (blocks_NNNNN.bin: nact, then bIDs), the order in which
+!                     the state dumps of that step are written.
 !   nsteps, ndump, dumps(:)  - steps after which U is written (0 = initial)
 !   tstop  - if > 0, stop (and write the state) after the first step that
 !            reaches time >= tstop
@@ -30,7 +37,7 @@ program wx_oracle_harness
   use hydro_solver, only : hydroSolver
   use coolingModule, only : cooling, find_coef
   use load_balance, only : loadBalance
-  use amr, only : meshlevel
+  use amr, only : meshlevel, markByPhysical, admesh
   use HLL, only : primfhll
   use HLLC, only : primfhllc
 #ifdef BFIELD
@@ -44,7 +51,8 @@ program wx_oracle_harness
   integer :: dumps(64) = -1
   integer :: op = 0
   real(8) :: tstop = -1.0
-  namelist /ctl/ mode, nsteps, ndump, dumps, op, tstop
+  logical :: amr = .false.
+  namelist /ctl/ mode, nsteps, ndump, dumps, op, tstop, amr
 
   integer :: s
 
@@ -76,6 +84,12 @@ program wx_oracle_harness
     call hydroSolver ()
     call updatePrims ()
     call cooling ()
+    if (amr) then
+      call write_flags (s)
+      call admesh ()
+      call loadBalance ()
+      call write_blocks (s)
+    end if
     if (rank == master) write (71,'(i8,3es26.17e3)') it, dt, time, nextout*dtout/t_sc
     if (dumpout) then
       nextout = nextout + 1
@@ -111,6 +125,41 @@ contains
     write (72) gamma, cv, CFL, visc_eta, d_sc, e_sc, t_sc, p_sc, l_sc, v_sc
     close (72)
   end subroutine write_topology
+
+  !-----------------------------------------------------------------------------
+  ! markByPhysical (source/amr.f90:142-230) of every active block, before admesh
+  subroutine write_flags (step)
+    integer, intent(in) :: step
+    character(len=32) :: fn
+    integer :: nb, nact, flag
+    nact = count(localBlocks(1:nbMaxProc) /= -1)
+    write (fn,'(a,i5.5,a)') 'flags_', step, '.bin'
+    open (unit=75, file=trim(fn), access='stream', form='unformatted', status='replace')
+    write (75) int(nact,4)
+    do nb=1,nbMaxProc
+      if (localBlocks(nb) /= -1) then
+        call markByPhysical (nb, localBlocks(nb), flag)
+        write (75) int(localBlocks(nb),4), int(flag,4)
+      end if
+    end do
+    close (75)
+  end subroutine write_flags
+
+  !-----------------------------------------------------------------------------
+  ! Active bIDs in slot order (the order of write_state)
+  subroutine write_blocks (step)
+    integer, intent(in) :: step
+    character(len=32) :: fn
+    integer :: nb, nact
+    nact = count(localBlocks(1:nbMaxProc) /= -1)
+    write (fn,'(a,i5.5,a)') 'blocks_', step, '.bin'
+    open (unit=76, file=trim(fn), access='stream', form='unformatted', status='replace')
+    write (76) int(nact,4)
+    do nb=1,nbMaxProc
+      if (localBlocks(nb) /= -1) write (76) int(localBlocks(nb),4)
+    end do
+    close (76)
+  end subroutine write_blocks
 
   !-----------------------------------------------------------------------------
   ! Interior U and PRIM of every active block, in slot order; per block the

@@ -1086,3 +1086,75 @@ void wxo_get_state(const wxo_sim *s, double *u, double *p) {
 }
 
 double *wxo_array(wxo_sim *s, int which) { return which == 0 ? s->U : which == 1 ? s->UP : s->PRIM; }
+
+void wxo_set_state_carry(wxo_sim *s, const double *u, const double *prim, const int *recompute) {
+  int nv = s->c.nv;
+  size_t n = (size_t)s->nblk * nv * NCELL;
+  memset(s->U, 0, n * sizeof(double));
+  memset(s->PRIM, 0, n * sizeof(double));
+  for (int b = 0; b < s->nblk; b++) {
+    for (int v = 0; v < nv; v++)
+      for (int k = 1; k <= NC; k++)
+        for (int j = 1; j <= NC; j++)
+          for (int i = 1; i <= NC; i++) {
+            size_t o = ((((size_t)b * nv + v) * NC + (k - 1)) * NC + (j - 1)) * NC + (i - 1);
+            size_t q = ((size_t)b * nv + v) * NCELL + cid(i, j, k);
+            s->U[q] = u[o];
+            if (!recompute[b]) s->PRIM[q] = prim[o];
+          }
+    if (recompute[b]) {
+      /* calcPrimsBlock(..., CELLS_PHYS) on the new block */
+      const double *Ub = s->U + (size_t)b * nv * NCELL;
+      double *Pb = s->PRIM + (size_t)b * nv * NCELL;
+      for (int k = 1; k <= NC; k++)
+        for (int j = 1; j <= NC; j++)
+          for (int i = 1; i <= NC; i++) prim_cell(s, Ub, Pb, cid(i, j, k));
+    }
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* markByPhysical (source/amr.f90:142-230), literally: cells in the    */
+/* reference's loop order (i outer, k inner), the pressure test before */
+/* the density test of each cell, early return on the first cell at or */
+/* above the refinement threshold.  maxgrad is the running maximum at  */
+/* the point of return (all cells when none triggers).                 */
+/* ------------------------------------------------------------------ */
+static double rel_grad(const double *A, int i, int j, int k, double two_dx) {
+  int q = cid(i, j, k);
+  double gradx = fabs(A[cid(i + 1, j, k)] - A[cid(i - 1, j, k)]) / A[q] / two_dx;
+  double grady = fabs(A[cid(i, j + 1, k)] - A[cid(i, j - 1, k)]) / A[q] / two_dx;
+  double gradz = fabs(A[cid(i, j, k + 1)] - A[cid(i, j, k - 1)]) / A[q] / two_dx;
+  double g = gradx > grady ? gradx : grady;
+  return g > gradz ? g : gradz;
+}
+
+static int mark_block(const wxo_sim *s, int b, double refine_thres, double coarse_thres, double *maxgrad_out) {
+  int nv = s->c.nv, ilev = s->lev[b];
+  const double *P5 = s->PRIM + ((size_t)b * nv + 4) * NCELL;
+  const double *P1 = s->PRIM + ((size_t)b * nv + 0) * NCELL;
+  double two_dx = 2 * s->dx[0][ilev - 1];
+  double maxgrad = 0.0;
+  for (int i = 1; i <= NC; i++)
+    for (int j = 1; j <= NC; j++)
+      for (int k = 1; k <= NC; k++) {
+        for (int pass = 0; pass < 2; pass++) {
+          double grad = rel_grad(pass == 0 ? P5 : P1, i, j, k, two_dx);
+          if (grad > maxgrad) maxgrad = grad;
+          if (grad >= refine_thres) {
+            *maxgrad_out = maxgrad;
+            return ilev == s->maxlev ? 0 /* FLAG_NONE */ : 1 /* FLAG_REFINE */;
+          }
+        }
+      }
+  *maxgrad_out = maxgrad;
+  return (maxgrad <= coarse_thres && ilev > 1) ? 2 /* FLAG_COARSE */ : 0;
+}
+
+void wxo_mark(const wxo_sim *s, double refine_thres, double coarse_thres, int *flags, double *maxgrad) {
+  for (int b = 0; b < s->nblk; b++) {
+    double m;
+    flags[b] = mark_block(s, b, refine_thres, coarse_thres, &m);
+    if (maxgrad) maxgrad[b] = m;
+  }
+}

@@ -66,6 +66,18 @@ void wxo_cooling(wxo_sim *s, double dt);
 /* raw arrays with ghosts, [nblk][nv][20][20][20]; which: 0=U,1=UP,2=PRIM */
 double *wxo_array(wxo_sim *s, int which);
 
+/* State after a mesh change (refineBlock / coarseBlocks, source/amr.f90:
+   284-384, 1430-1650): interior U and PRIM [nblk][nv][16^3], ghosts
+   zeroed; PRIM taken as given except for blocks with recompute[b] != 0
+   (new coarse blocks), where it is calcPrimsBlock of U on the interior. */
+void wxo_set_state_carry(wxo_sim *s, const double *u, const double *prim, const int *recompute);
+
+/* markByPhysical (source/amr.f90:142-230) of every block on the current
+   PRIM (with its ghost layer): flags FLAG_NONE 0 / FLAG_REFINE 1 /
+   FLAG_COARSE 2; maxgrad (may be NULL) the running maximum gradient at the
+   point the reference returns. */
+void wxo_mark(const wxo_sim *s, double refine_thres, double coarse_thres, int *flags, double *maxgrad);
+
 /* Per-function entry points (single cell / single interface). */
 int wxo_flow2prim(const wxo_config *c, const double *u, double *p);
 void wxo_prim2flow(const wxo_config *c, const double *p, double *u);

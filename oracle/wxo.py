@@ -52,6 +52,8 @@ def lib():
         L.wxo_godunov.argtypes = [C.c_void_p, C.c_int, C.c_double]
         L.wxo_viscous.argtypes = [C.c_void_p]
         L.wxo_cooling.argtypes = [C.c_void_p, C.c_double]
+        L.wxo_set_state_carry.argtypes = [C.c_void_p, _dp, _dp, _ip]
+        L.wxo_mark.argtypes = [C.c_void_p, C.c_double, C.c_double, _ip, _dp]
         L.wxo_array.restype = _dp
         L.wxo_array.argtypes = [C.c_void_p, C.c_int]
         L.wxo_flow2prim.restype = C.c_int
@@ -135,6 +137,27 @@ class Sim:
         U = np.ascontiguousarray(U, dtype=np.float64)
         assert U.shape == (self.nblk, self.w.nv, 16, 16, 16)
         lib().wxo_set_state(self._h, _ptr(U))
+
+    def set_state_carry(self, U, P, recompute):
+        """U, PRIM after a mesh change; PRIM recomputed where recompute != 0."""
+        U = np.ascontiguousarray(U, dtype=np.float64)
+        P = np.ascontiguousarray(P, dtype=np.float64)
+        rc = np.ascontiguousarray(recompute, dtype=np.int32)
+        assert U.shape == P.shape == (self.nblk, self.w.nv, 16, 16, 16) and rc.shape == (self.nblk,)
+        lib().wxo_set_state_carry(self._h, _ptr(U), _ptr(P), rc.ctypes.data_as(_ip))
+
+    def mark(self, refine_thres, coarse_thres):
+        """markByPhysical of every block: (flags, maxgrad)."""
+        flags = np.zeros(self.nblk, dtype=np.int32)
+        mg = np.zeros(self.nblk)
+        lib().wxo_mark(self._h, refine_thres, coarse_thres, flags.ctypes.data_as(_ip), _ptr(mg))
+        return flags, mg
+
+    def counters(self):
+        return self.it.value, self.dt.value, self.time.value, self.nextout.value
+
+    def set_counters(self, it, dt, time, nextout):
+        self.it.value, self.dt.value, self.time.value, self.nextout.value = it, dt, time, nextout
 
     def get_state(self):
         u = np.empty((self.nblk, self.w.nv, 16, 16, 16))

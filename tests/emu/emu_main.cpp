@@ -1,7 +1,9 @@
 // Test infrastructure: drives the library's C ABI (include/walicxe_hip.h) on
 // the CPU build of tests/emu for the cases named on the command line: each
 // pair is an input file written by tests/test_emulated_kernels.py and a step
-// count; the final state goes to <input>.out.
+// count; the final state goes to <input>.out.  A remesh count of -1 in the
+// input asks for wx_admesh after every step instead (final block list to
+// <input>.blocks).
 #include <cstdio>
 #include <cstdlib>
 #include <vector>
@@ -33,7 +35,10 @@ static int run_case(const char *path, int nsteps) {
   // optional: a second block list to move the state to (wx_remesh) after the run
   int nb2 = 0;
   std::vector<int> bids2;
-  if (fread(&nb2, 4, 1, f) == 1 && nb2 > 0) {
+  double thres[2] = {0.3, 0.03};
+  if (fread(&nb2, 4, 1, f) == 1 && nb2 == -1) {
+    fread(thres, 8, 2, f);
+  } else if (nb2 > 0) {
     bids2.resize(nb2);
     fread(bids2.data(), 4, nb2, f);
   }
@@ -45,7 +50,21 @@ static int run_case(const char *path, int nsteps) {
   rc = wx_set_mesh(ctx, nroot, maxlev, dx.data(), nblk, bids.data()); if (rc < 0) { printf("mesh %d %s\n", rc, wx_last_error()); return 1; }
   rc = wx_upload_state(ctx, U.data(), WX_LAYOUT_BLOCKS, 0, nullptr); if (rc < 0) { printf("upload %d %s\n", rc, wx_last_error()); return 1; }
   int it = 0, nextout = 1; double dt = 0, time = 0;
-  if (cfg.form == WX_FORM_PLAIN_FP64) {
+  if (nb2 == -1) {
+    std::vector<int> nb(8 * 4096);
+    for (int n = 0; n < nsteps && rc >= 0; n++) {
+      if (cfg.form == WX_FORM_PLAIN_FP64) rc = wx_step(ctx, &it, &dt, &time, &nextout, dtout_tsc[0], dtout_tsc[1]);
+      else rc = wx_run(ctx, 1, &it, &dt, &time, &nextout, dtout_tsc[0], dtout_tsc[1]);
+      if (rc < 0) break;
+      rc = wx_admesh(ctx, thres[0], thres[1], (int)nb.size(), &nblk, nb.data());
+      if (rc < 0) { printf("admesh %d %s\n", rc, wx_last_error()); return 1; }
+      printf("step %d blocks %d\n", n + 1, nblk); fflush(stdout);
+    }
+    bids.assign(nb.begin(), nb.begin() + nblk);
+    U.resize((size_t)nblk * nv * 4096);
+    std::string fn = std::string(path) + ".blocks";
+    FILE *fb = fopen(fn.c_str(), "wb"); fwrite(bids.data(), 4, nblk, fb); fclose(fb);
+  } else if (cfg.form == WX_FORM_PLAIN_FP64) {
     for (int n = 0; n < nsteps && rc >= 0; n++)
       rc = wx_step(ctx, &it, &dt, &time, &nextout, dtout_tsc[0], dtout_tsc[1]);
   } else {

@@ -11,6 +11,11 @@ Fixtures (numpy .npz, no pickles):
                         seeded sample of 4096 full cell vectors of U and PRIM,
                         keyed by reference block ID; a checksum of the initial
                         condition so that drift of the generator is caught.
+  amr_<workload>.npz    the same with the reference's mesh adaptation after
+                        every step (admesh): per step the markByPhysical flag
+                        of every block (f<s>_bids, f<s>_flags) and the block
+                        list after admesh in slot order (b<s>), the state
+                        summaries of the dumped steps in that order.
   unit_<build>.npz      reference routine outputs on seeded inputs
                         (flow2prim, prim2flow, prim2fluxes, limiter, Riemann
                         solvers, wave speeds, find_coef, calcTemp).
@@ -43,6 +48,10 @@ TRAJ = {
     "winds4": (5, [5]),
 }
 NSAMPLE = 4096
+# workloads run with admesh after every step: (steps, dumped steps)
+AMR_TRAJ = {
+    "amr_adapt": (8, [2, 8]),
+}
 
 
 def summarize(w, bids, U, P, seed):
@@ -79,6 +88,27 @@ def make_traj(name):
             out["d%d_%s" % (d, k)] = v
     np.savez_compressed(os.path.join(HERE, "traj_%s.npz" % name), **out)
     print("traj %-14s steps=%d dumps=%s (%.1fs)" % (name, n, dumps, time.time() - t0), flush=True)
+
+
+def make_amr_traj(name):
+    w = W.get(name)
+    n, dumps = AMR_TRAJ[name]
+    t0 = time.time()
+    r = refrun.run(w, n, dumps, timeout=7200, amr=True)
+    out = dict(steps=r["steps"], ic_bids=np.asarray(r["ic_blocks"], dtype=np.int32),
+               dumps=np.asarray(dumps, dtype=np.int32),
+               ic_sum=np.asarray([np.sum(r["ic_U"]), np.sum(np.abs(r["ic_U"]))]))
+    for s in range(1, n + 1):
+        b, f = r["flags"][s]
+        out["f%d_bids" % s] = b.astype(np.int32)
+        out["f%d_flags" % s] = f.astype(np.int32)
+        out["b%d" % s] = r["blocks"][s].astype(np.int32)
+    for d in dumps:
+        u, p = r["states"][d]
+        for k, v in summarize(w, r["blocks"][d], u, p, 1000 * w.seed + d).items():
+            out["d%d_%s" % (d, k)] = v
+    np.savez_compressed(os.path.join(HERE, "amr_%s.npz" % name), **out)
+    print("amr  %-14s steps=%d dumps=%s (%.1fs)" % (name, n, dumps, time.time() - t0), flush=True)
 
 
 def rand_prims(rng, n, nv, mhd):
@@ -190,9 +220,13 @@ def main(argv):
     for n in names:
         if n.startswith("unit:"):
             make_unit(n[5:], UNITS[n[5:]])
+        elif n.startswith("amr:"):
+            make_amr_traj(n[4:])
         else:
             make_traj(n)
     if not argv:
+        for n in AMR_TRAJ:
+            make_amr_traj(n)
         for n, ops in UNITS.items():
             make_unit(n, ops)
 

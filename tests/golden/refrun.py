@@ -56,14 +56,24 @@ def read_state(path, nblk, nv_ref):
     return a.reshape(nblk, nv_ref, 16, 16, 16)
 
 
+def read_ints(path):
+    a = np.fromfile(path, dtype="<i4")
+    return a[1:1 + int(a[0])].copy()
+
+
 def run(w, nsteps, dumps, workdir=None, unit_op=None, unit_in=None, timeout=3600, tstop=-1.0, variant="",
-        nprocs=1):
+        nprocs=1, amr=False):
     """Run the harness; returns dict with topo, steps (it, dt, time, tout),
     and states {step: (U, PRIM)} with the passive scalar stripped.  With
     nprocs > 1 the multi-rank build runs under mpiexec and only the step log
-    comes back (no topology, no states)."""
+    comes back (no topology, no states).  With amr the adaptive-mesh build
+    runs admesh after every step; then also flags {step: (bids, flags)}
+    (markByPhysical before admesh) and blocks {step: bids after admesh, in
+    the order of that step's state dump}."""
     if nprocs > 1:
         variant = variant + "_np%d" % nprocs
+    if amr:
+        variant = variant + "_amr"
     exe = harness_path(w.name + variant)
     if not os.path.exists(exe):
         raise FileNotFoundError(exe)
@@ -84,8 +94,8 @@ def run(w, nsteps, dumps, workdir=None, unit_op=None, unit_in=None, timeout=3600
     write_ic_with_passive(os.path.join(wd, "ic.bin"), w, blocks, U)
     dl = ",".join(str(d) for d in dumps)
     with open(os.path.join(wd, "harness.nml"), "w") as f:
-        f.write("&ctl\n mode='run'\n nsteps=%d\n ndump=%d\n dumps=%s\n tstop=%r\n/\n"
-                % (nsteps, len(dumps), dl or "-1", float(tstop)))
+        f.write("&ctl\n mode='run'\n nsteps=%d\n ndump=%d\n dumps=%s\n tstop=%r\n amr=%s\n/\n"
+                % (nsteps, len(dumps), dl or "-1", float(tstop), ".true." if amr else ".false."))
     cmd = [exe] if nprocs == 1 else [MPIEXEC, "-n", str(nprocs), exe]
     r = subprocess.run(cmd, cwd=wd, timeout=timeout, capture_output=True, text=True)
     if r.returncode != 0:
@@ -102,11 +112,20 @@ def run(w, nsteps, dumps, workdir=None, unit_op=None, unit_in=None, timeout=3600
     states = {}
     if tstop > 0 and len(steps) and int(steps[-1, 0]) not in dumps:
         dumps = list(dumps) + [int(steps[-1, 0])]
+    flags, blocks_after = {}, {}
+    if amr:
+        for s in range(1, len(steps) + 1):
+            fl = np.fromfile(os.path.join(wd, "flags_%05d.bin" % s), dtype="<i4")[1:].reshape(-1, 2)
+            flags[s] = (fl[:, 0].copy(), fl[:, 1].copy())
+            blocks_after[s] = read_ints(os.path.join(wd, "blocks_%05d.bin" % s))
     for d in dumps:
-        u = read_state(os.path.join(wd, "u_%05d.bin" % d), nblk, w.nv + 1)[:, : w.nv]
-        p = read_state(os.path.join(wd, "prim_%05d.bin" % d), nblk, w.nv + 1)[:, : w.nv]
+        nb = len(blocks_after[d]) if (amr and d > 0) else nblk
+        u = read_state(os.path.join(wd, "u_%05d.bin" % d), nb, w.nv + 1)[:, : w.nv]
+        p = read_state(os.path.join(wd, "prim_%05d.bin" % d), nb, w.nv + 1)[:, : w.nv]
         states[d] = (u.copy(), p.copy())
     out = dict(topo=topo, steps=steps, states=states, ic_blocks=blocks, ic_U=U, dumps=list(dumps))
+    if amr:
+        out.update(flags=flags, blocks=blocks_after)
     if own:
         import shutil
         shutil.rmtree(wd, ignore_errors=True)

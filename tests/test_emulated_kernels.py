@@ -49,7 +49,7 @@ def _build():
     return exe
 
 
-def _write_input(path, w, form, remesh_to=None):
+def _write_input(path, w, form, remesh_to=None, admesh=None):
     slots, blocks, U0 = W.initial_state(w)
     cfg = hip.make_config(w, form)
     raw = bytes(cfg)
@@ -68,6 +68,8 @@ def _write_input(path, w, form, remesh_to=None):
         if remesh_to is not None:
             f.write(struct.pack("<i", len(remesh_to)))
             f.write(np.asarray(remesh_to, dtype="<i4").tobytes())
+        elif admesh is not None:
+            f.write(struct.pack("<i2d", -1, *admesh))
     return blocks, U0
 
 
@@ -115,3 +117,27 @@ def test_emulation_reproduces_fused_fp64(emu, tmp_path, oracle):
     uo, _ = o.get_state()
     ue = np.fromfile(p + ".out").reshape(uo.shape)
     assert np.array_equal(ue, uo)
+
+
+def test_emulated_admesh_reproduces_the_oracle(emu, tmp_path, oracle):
+    """wx_admesh after every step (device flagging, host plan, remesh with
+    PRIM carried) in the plain FP64 form, against the oracle's restatement of
+    the reference's admesh: same final block set, bit-identical state.  The
+    fused form runs the same calls for the bounds check."""
+    from oracle import amr
+    w = W.get("amr_adapt")
+    nsteps = 3
+    p = str(tmp_path / "adapt_plain.bin")
+    _write_input(p, w, hip.PLAIN_FP64, admesh=(0.3, 0.03))
+    q = str(tmp_path / "adapt_fused.bin")
+    _write_input(q, w, hip.FUSED_FP32, admesh=(0.3, 0.03))
+    out = _run(emu, [p, str(nsteps), q, str(nsteps)], str(tmp_path))
+    assert out.count("it ") == 2, out
+    o = amr.trajectory(w, nsteps, [nsteps])
+    ob = [int(b) for b in o["blocks"][nsteps]]
+    eb = [int(b) for b in np.fromfile(p + ".blocks", dtype=np.int32)]
+    assert sorted(eb) == sorted(ob)
+    uo, _ = o["states"][nsteps]
+    ue = np.fromfile(p + ".out").reshape((len(eb),) + uo.shape[1:])
+    pos = {b: n for n, b in enumerate(eb)}
+    assert np.array_equal(ue[[pos[b] for b in ob]], uo)

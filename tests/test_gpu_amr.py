@@ -1,0 +1,133 @@
+"""GPU parity of mesh adaptation: wx_mark_blocks (markByPhysical,
+source/amr.f90:142-230, on the device) and wx_admesh (admesh, :44-133: the
+flags, checkProximity, refine / coarsen lists and the state moved on the
+device) against the reference's own adaptive run
+(tests/golden/amr_amr_adapt.npz) and the oracle.
+
+Bars: flags and block lists identical to the reference's at every step (the
+closest block gradient stays 3% away from a threshold on this run, far beyond
+any rounding); the plain FP64 state within the plain form's cooling bar
+(relative 1e-12: ocml vs glibc exp/log10/pow); in the fused form the device
+flags and gradients equal the oracle's on the same state, the new list equals
+wx_admesh_plan's and the moved state equals the host regrid, bit for bit.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import compare_dump, ic_in_order, load_golden
+from walicxe3d_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+RT, CT = 0.3, 0.03
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _by_bid(bids, values, order):
+    pos = {int(b): n for n, b in enumerate(bids)}
+    return np.asarray(values)[[pos[int(b)] for b in order]]
+
+
+def test_plain_admesh_matches_the_reference():
+    from walicxe3d_amd import hip
+    fx = load_golden("amr_amr_adapt.npz")
+    w = W.get("amr_adapt")
+    b0 = fx["ic_bids"]
+    nsteps = len(fx["steps"])
+    dumps = [int(d) for d in fx["dumps"]]
+    say("case plain admesh amr_adapt steps", nsteps, "blocks", len(b0))
+    g = hip.Walicxe3D(w, form=hip.PLAIN_FP64, bids=b0)
+    g.upload(ic_in_order(w, b0))
+    for s in range(1, nsteps + 1):
+        g.step()
+        assert g.dt.value == pytest.approx(fx["steps"][s - 1, 1], rel=1e-13)
+        f, grad = g.markByPhysical(RT, CT)
+        assert np.array_equal(_by_bid(g.bids, f, fx["f%d_bids" % s]), fx["f%d_flags" % s]), s
+        new = g.admesh(RT, CT)
+        assert sorted(new.tolist()) == sorted(fx["b%d" % s].tolist()), s
+        if s in dumps:
+            u, p = g.download()
+            order = fx["b%d" % s]
+            dev_sum, dev_s = compare_dump(w, fx, s, _by_bid(new, u, order), _by_bid(new, p, order), order)
+            say("  step", s, "blocks", len(new), "dev", dev_sum, dev_s)
+            assert dev_sum <= 1e-12 and dev_s <= 1e-12, (s, dev_sum, dev_s)
+    g.close()
+
+
+def _oracle_mark(oracle, w, bids, u_ghosts, u_now):
+    """markByPhysical on PRIM whose ghost layer is one exchange of u_ghosts
+    and whose interior is flow2prim of u_now."""
+    o = oracle.Sim(w, bids)
+    o.set_state(u_ghosts)
+    o.boundary(1, 0)
+    o.calc_prims(0, 2)
+    n = oracle.Sim(w, bids)
+    n.set_state(u_now)
+    o.array(2)[:, :, 2:18, 2:18, 2:18] = n.array(2)[:, :, 2:18, 2:18, 2:18]
+    return o.mark(RT, CT)
+
+
+@pytest.mark.parametrize("record", [True, False])
+def test_fused_mark_and_admesh_match_oracle_and_plan(oracle, record):
+    """With wx_record_ghosts before each step the ghost layer is the one
+    the reference's markByPhysical reads (the step's first exchange) and the
+    decisions are the reference's; without it the current state is
+    exchanged.  Either way the device equals the oracle on the same data."""
+    from walicxe3d_amd import hip, regrid
+    fx = load_golden("amr_amr_adapt.npz")
+    w = W.get("amr_adapt")
+    m = w.mesh()
+    b0 = fx["ic_bids"]
+    say("case fused admesh amr_adapt record", record)
+    g = hip.Walicxe3D(w, form=hip.FUSED_FP32, bids=b0)
+    g.upload(ic_in_order(w, b0))
+    for s in range(1, 5):
+        u0 = g.download(prim=False)
+        if record:
+            g.record_ghosts()
+        g.run(1)
+        u = g.download(prim=False)
+        f, grad = g.markByPhysical(RT, CT)
+        fo, go = _oracle_mark(oracle, w, g.bids, u0 if record else u, u)
+        assert np.array_equal(f, fo), s
+        # the reference returns at the first cell at or above the refinement
+        # threshold (REFINE, or NONE at the maximum level): its running
+        # maximum is the whole-block maximum only where it stays below
+        full = go < RT
+        assert np.array_equal(grad[full], go[full]), s
+        assert np.all(grad[~full] >= go[~full])
+        old = g.bids.copy()
+        plan, _ = hip.admesh_plan(m, old, f)
+        new = g.admesh(RT, CT)
+        assert np.array_equal(new, plan), s
+        if record:
+            # the reference's decisions (FP32 bulk arithmetic moves no block
+            # gradient across a threshold on this run)
+            assert sorted(new.tolist()) == sorted(fx["b%d" % s].tolist()), s
+        if not np.array_equal(new, old):
+            assert np.array_equal(g.download(prim=False), regrid.remap(m, old, u, new)), s
+        say("  step", s, "blocks", len(new))
+    g.close()
+
+
+def test_admesh_over_capacity_leaves_the_state():
+    from walicxe3d_amd import hip
+    w = W.get("amr_adapt")
+    b0 = np.asarray(w.blocks(), dtype=np.int32)
+    say("case admesh capacity", len(b0))
+    g = hip.Walicxe3D(w, form=hip.FUSED_FP32, bids=b0)
+    g.upload(ic_in_order(w, b0))
+    g.run(1)
+    u = g.download(prim=False)
+    with pytest.raises(hip.WxError):
+        g.admesh(RT, CT, cap=8)
+    assert np.array_equal(g.download(prim=False), u)
+    g.bids, g.nblk = b0, len(b0)
+    g.run(1)                                         # the context still steps
+    g.close()

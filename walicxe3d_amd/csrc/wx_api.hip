@@ -4,11 +4,13 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <string>
 #include <unordered_map>
+#include <unordered_set>
 #include <vector>
 
 #include "wx_device.h"
@@ -50,6 +52,7 @@ struct Ctx {
   double *dU = nullptr, *dUP = nullptr, *dP = nullptr, *dF = nullptr, *dTmp = nullptr;
   unsigned long long *d_scal = nullptr;  // [0] dt bits, [1] floor count
   bool have_state = false;
+  bool ghosts_recorded = false;      // fused form: PRIM ghost layer of wx_record_ghosts in dP
   FusedState fused;
 };
 
@@ -64,6 +67,7 @@ void free_mesh(Ctx *c) {
   c->dU = c->dUP = c->dP = c->dF = c->dTmp = nullptr;
   fused_free(c->fused);
   c->have_state = false;
+  c->ghosts_recorded = false;
 }
 
 // ---------------------------------------------------------------------------
@@ -424,6 +428,205 @@ static int remap_table(Ctx *c, int nblk, const int *bids, std::vector<RemapSrc> 
   return WX_OK;
 }
 
+// PRIM of the new blocks that the reference carries through a mesh change:
+// copied blocks keep theirs, refined children take their father's by the
+// same injection as U (refineBlock, source/amr.f90:355-365); new coarse
+// blocks (kind 2) keep the PRIM computed from their averaged U.
+__global__ void __launch_bounds__(256) k_carry_prims(int nv, const RemapSrc *map, const double *src, double *P) {
+  const int b = blockIdx.x, v = blockIdx.y;
+  if (map[b].kind == 2) return;
+  const int i = threadIdx.x & 15, j = threadIdx.x >> 4;
+  const double *S = src + ((size_t)b * nv + v) * NINT;
+  double *D = P + ((size_t)b * nv + v) * NCELL;
+  for (int k = 0; k < NC; k++) D[cid(i + 1, j + 1, k + 1)] = S[(k * NC + j) * NC + i];
+}
+
+// ---------------------------------------------------------------------------
+// Mesh adaptation, admesh (source/amr.f90:44-133) on one rank: the flags
+// come from the device (k_mark); the proximity rule, the sibling families and
+// the new block list are host work on block IDs, with a bID -> index map in
+// place of the reference's scans of the global block list.
+// ---------------------------------------------------------------------------
+enum { FLAG_NONE = 0, FLAG_REFINE = 1, FLAG_COARSE = 2 };   // constants.f90:76-78
+
+static long amr_father(const Numbering &N, long bid) {
+  const int lev = N.level(bid);
+  if (lev <= 1) return -1;
+  int c[3];
+  N.coords(bid, lev, c);
+  int f[3] = {(c[0] + 1) / 2, (c[1] + 1) / 2, (c[2] + 1) / 2};
+  return N.bid(lev - 1, f);
+}
+
+// children (source/amr.f90:882-909): x fastest, then y, then z
+static long amr_child(const Numbering &N, long bid, int q) {
+  const int lev = N.level(bid);
+  int c[3];
+  N.coords(bid, lev, c);
+  int ch[3] = {2 * c[0] - 1 + (q & 1), 2 * c[1] - 1 + ((q >> 1) & 1), 2 * c[2] - 1 + ((q >> 2) & 1)};
+  return N.bid(lev + 1, ch);
+}
+
+// neighbors (source/amr.f90:539-631) over the active set: NEIGH_BOUNDARY at
+// every domain face (neighborLevel, :439-523, does not wrap periodic faces);
+// for NEIGH_SAME and NEIGH_COARSER *id is the neighbour, for NEIGH_FINER the
+// inactive same-level block whose children face bid.
+static int amr_neighbour(const Numbering &N, const std::unordered_map<long, int> &act, long bid, int dir, long *id) {
+  const int lev = N.level(bid);
+  int c[3];
+  N.coords(bid, lev, c);
+  long n[3];
+  N.nb(lev, n);
+  const int ax = dir >> 1;
+  c[ax] += (dir & 1) ? 1 : -1;
+  if (c[ax] < 1 || c[ax] > n[ax]) return NEIGH_BOUNDARY;
+  const long nid = N.bid(lev, c);
+  *id = nid;
+  if (act.count(nid)) return NEIGH_SAME;
+  const long fid = amr_father(N, nid);
+  if (fid != -1 && act.count(fid)) {
+    *id = fid;
+    return NEIGH_COARSER;
+  }
+  return NEIGH_FINER;
+}
+
+// checkProximity(maxlev - 1) (source/amr.f90:1109-1254) on `flag`, then the
+// block list after refineBlock (:284-384) of every block flagged for
+// refinement and coarseBlocks (:1430-1650) of every family whose 8 members
+// are flagged for coarsening.  The new list keeps the old order: the 8
+// children in place of their father, a father in place of the first of its
+// children in the list.  Within one level the proximity result does not
+// depend on the order of the blocks: a block's own test reads only flags of
+// finer levels, which are final by then, and the same-level changes only
+// turn COARSE into NONE.
+static int amr_plan(const Numbering &N, const std::vector<int> &bids, std::vector<int> &flag, std::vector<int> &out) {
+  const size_t nb = bids.size();
+  std::unordered_map<long, int> act;
+  act.reserve(2 * nb);
+  std::vector<std::vector<int>> by_level(N.maxlev + 1);
+  for (size_t n = 0; n < nb; n++) {
+    const int lev = N.level(bids[n]);
+    if (lev < 0) return fail(WX_ETOPO, "block ID outside the mesh: " + std::to_string(bids[n]));
+    if (!act.emplace(bids[n], (int)n).second) return fail(WX_ETOPO, "duplicate block ID " + std::to_string(bids[n]));
+    if (flag[n] < FLAG_NONE || flag[n] > FLAG_COARSE) return fail(WX_EINVAL, "unknown refinement flag");
+    by_level[lev].push_back((int)n);
+  }
+  for (int level = N.maxlev - 1; level >= 1; level--)
+    for (int n : by_level[level]) {
+      const long b = bids[n];
+      const int bflag = flag[n];
+      if (bflag == FLAG_REFINE) {
+        // same-level neighbours stop coarsening, coarser ones must refine
+        for (int dir = 0; dir < 6; dir++) {
+          long id = -1;
+          const int t = amr_neighbour(N, act, b, dir, &id);
+          if (t == NEIGH_SAME) {
+            int &f = flag[act.at(id)];
+            if (f == FLAG_COARSE) f = FLAG_NONE;
+          } else if (t == NEIGH_COARSER) {
+            flag[act.at(id)] = FLAG_REFINE;
+          }
+        }
+      } else if (bflag == FLAG_COARSE) {
+        // no coarsening next to a finer family with a member not coarsening
+        bool inhibited = false;
+        for (int dir = 0; dir < 6 && !inhibited; dir++) {
+          long id = -1;
+          if (amr_neighbour(N, act, b, dir, &id) != NEIGH_FINER) continue;
+          for (int q = 0; q < 8 && !inhibited; q++) {
+            auto it = act.find(amr_child(N, id, q));
+            if (it != act.end() && flag[it->second] != FLAG_COARSE) {
+              flag[n] = FLAG_NONE;
+              inhibited = true;
+            }
+          }
+        }
+      }
+    }
+  std::unordered_map<long, int> family;   // father -> members flagged
+  for (size_t n = 0; n < nb; n++)
+    if (flag[n] == FLAG_COARSE) {
+      const long f = amr_father(N, bids[n]);
+      if (f != -1) family[f]++;
+    }
+  out.clear();
+  out.reserve(nb);
+  std::unordered_set<long> placed;
+  for (size_t n = 0; n < nb; n++) {
+    const long b = bids[n];
+    if (flag[n] == FLAG_REFINE) {
+      if (N.level(b) >= N.maxlev)
+        return fail(WX_ETOPO, "block " + std::to_string(b) + " is flagged for refinement at the maximum level");
+      for (int q = 0; q < 8; q++) out.push_back((int)amr_child(N, b, q));
+      continue;
+    }
+    if (flag[n] == FLAG_COARSE) {
+      const long f = amr_father(N, b);
+      auto it = family.find(f);
+      if (f != -1 && it != family.end() && it->second == 8) {
+        if (placed.insert(f).second) out.push_back((int)f);
+        continue;
+      }
+    }
+    out.push_back((int)b);
+  }
+  return WX_OK;
+}
+
+// face table of a block list without touching a context (2:1 balance and
+// the periodic partners checked as wx_set_mesh does)
+static int check_mesh(const Ctx *c, const std::vector<int> &bids) {
+  Ctx t;
+  t.cfg = c->cfg;
+  t.maxlev = c->maxlev;
+  for (int d = 0; d < 3; d++) t.nroot[d] = c->nroot[d];
+  t.nblk = (int)bids.size();
+  return build_topology(&t, bids.data());
+}
+
+// The fused form keeps no ghost layers: the depth-1 exchange of its current
+// state into the plain arrays, PRIM on the face ghosts (dP's ghost layer).
+static int fused_ghost_prims(Ctx *c) {
+  if (fused_download(c->fused, c->dc, c->dU, c->dP, c->st)) return fail(WX_EHIP, "fused state read-back failed");
+  plain_boundary(c->dc, c->dU, 1, c->st);
+  plain_prims(c->dc, c->dU, c->dP, WX_CELLS_GHOST, nullptr, c->st);
+  return launch_check("ghost prims");
+}
+
+// markByPhysical flags of every block of the context
+static int mark_blocks(Ctx *c, double refine_thres, double coarse_thres, int *flags, double *maxgrad) {
+  if (is_fused(c)) {
+    if (c->ghosts_recorded) {
+      // interior PRIM of the current state; the ghost layer stays the recorded one
+      if (fused_download(c->fused, c->dc, c->dU, c->dP, c->st))
+        return fail(WX_EHIP, "fused state read-back failed");
+    } else {
+      int rc = fused_ghost_prims(c);
+      if (rc != WX_OK) return rc;
+    }
+  }
+  std::vector<double> m(c->nblk);
+  double *d_m = nullptr;
+  HIPCHK(hipMalloc(&d_m, m.size() * sizeof(double)));
+  plain_mark(c->dc, c->dP, d_m, c->st);
+  int rc = launch_check("mark");
+  if (rc == WX_OK && hipMemcpyAsync(m.data(), d_m, m.size() * sizeof(double), hipMemcpyDeviceToHost, c->st) != hipSuccess)
+    rc = fail(WX_EHIP, "mark read-back failed");
+  if (rc == WX_OK && hipStreamSynchronize(c->st) != hipSuccess) rc = fail(WX_EHIP, "mark failed");
+  hipFree(d_m);
+  if (rc != WX_OK) return rc;
+  for (int n = 0; n < c->nblk; n++) {
+    const int lev = c->info[n].level;
+    if (m[n] >= refine_thres)
+      flags[n] = lev < c->maxlev ? FLAG_REFINE : FLAG_NONE;
+    else
+      flags[n] = (m[n] <= coarse_thres && lev > 1) ? FLAG_COARSE : FLAG_NONE;
+    if (maxgrad) maxgrad[n] = m[n];
+  }
+  return WX_OK;
+}
+
 extern "C" {
 
 const char *wx_last_error(void) { return g_err.c_str(); }
@@ -537,6 +740,7 @@ static int upload_device(Ctx *c, const double *d_in) {
   HIPCHK(hipStreamSynchronize(c->st));
   if (rc != WX_OK) return rc;
   c->have_state = true;
+  c->ghosts_recorded = false;
   if (is_fused(c)) return fused_upload(c->fused, c->dc, c->dU, c->st) ? WX_EHIP : WX_OK;
   long bad = 0;
   rc = update_prims(c, &bad);
@@ -793,30 +997,106 @@ int wx_remesh(void *ctx, int nblk, const int *bids) {
   rc = remap_table(c, nblk, bids, table);
   if (rc != WX_OK) return rc;              // the current mesh and state are untouched
   const int nv = c->cfg.nv;
+  // the plain form also carries PRIM (its interior pressure is the one
+  // cooling left, not the one U gives back)
+  const bool carry = !is_fused(c);
   HIPCHK(hipStreamSynchronize(c->st));
   if (is_fused(c) && fused_download(c->fused, c->dc, c->dU, c->dP, c->st))
     return fail(WX_EHIP, "fused state read-back failed");
   size_t nold = (size_t)c->nblk * nv * NINT, nnew = (size_t)nblk * nv * NINT;
-  double *d_old = nullptr, *d_new = nullptr;
+  double *d_old = nullptr, *d_new = nullptr, *d_newp = nullptr;
   RemapSrc *d_map = nullptr;
   HIPCHK(hipMalloc(&d_old, nold * sizeof(double)));
   HIPCHK(hipMalloc(&d_new, nnew * sizeof(double)));
+  if (carry) HIPCHK(hipMalloc(&d_newp, nnew * sizeof(double)));
   HIPCHK(hipMalloc(&d_map, table.size() * sizeof(RemapSrc)));
-  hipLaunchKernelGGL(k_gather, dim3((unsigned)((nold + 255) / 256)), dim3(256), 0, c->st, nv, c->nblk, c->dU, d_old);
   HIPCHK(hipMemcpyAsync(d_map, table.data(), table.size() * sizeof(RemapSrc), hipMemcpyHostToDevice, c->st));
+  hipLaunchKernelGGL(k_gather, dim3((unsigned)((nold + 255) / 256)), dim3(256), 0, c->st, nv, c->nblk, c->dU, d_old);
   hipLaunchKernelGGL(k_remap, dim3(nblk, nv), dim3(256), 0, c->st, nv, d_map, d_old, d_new);
+  if (carry) {
+    hipLaunchKernelGGL(k_gather, dim3((unsigned)((nold + 255) / 256)), dim3(256), 0, c->st, nv, c->nblk, c->dP, d_old);
+    hipLaunchKernelGGL(k_remap, dim3(nblk, nv), dim3(256), 0, c->st, nv, d_map, d_old, d_newp);
+  }
   rc = launch_check("remesh");
   HIPCHK(hipStreamSynchronize(c->st));
   hipFree(d_old);
-  hipFree(d_map);
   if (rc == WX_OK) {
     const std::vector<double> dxyz = c->dxyz;
     int nroot[3] = {c->nroot[0], c->nroot[1], c->nroot[2]};
     rc = wx_set_mesh(ctx, nroot, c->maxlev, dxyz.data(), nblk, bids);
   }
-  if (rc == WX_OK) rc = upload_device(c, d_new);
+  if (rc >= WX_OK) rc = upload_device(c, d_new);
+  if (rc >= WX_OK && carry) {
+    hipLaunchKernelGGL(k_carry_prims, dim3(nblk, nv), dim3(256), 0, c->st, nv, d_map, d_newp, c->dP);
+    int rc2 = launch_check("remesh prims");
+    if (rc2 == WX_OK && hipStreamSynchronize(c->st) != hipSuccess) rc2 = fail(WX_EHIP, "remesh prims failed");
+    if (rc2 != WX_OK) rc = rc2;
+  }
+  hipFree(d_map);
   hipFree(d_new);
+  hipFree(d_newp);
   return rc;
+}
+
+int wx_admesh_plan(const int nroot[3], int maxlev, int nblk, const int *bids, const int *flags, int *flags_out,
+                   int cap, int *nblk_new, int *bids_new) {
+  if (!nroot || !bids || !flags || !nblk_new || nblk < 1 || maxlev < 1 || maxlev > MAXLEV)
+    return fail(WX_EINVAL, "bad argument");
+  Numbering N;
+  N.init(nroot, maxlev);
+  std::vector<int> b(bids, bids + nblk), f(flags, flags + nblk), out;
+  int rc = amr_plan(N, b, f, out);
+  if (rc != WX_OK) return rc;
+  if (flags_out) std::copy(f.begin(), f.end(), flags_out);
+  *nblk_new = (int)out.size();
+  if ((int)out.size() > cap || !bids_new)
+    return fail(WX_ENOMEM, "new block list (" + std::to_string(out.size()) + " blocks) exceeds the capacity");
+  std::copy(out.begin(), out.end(), bids_new);
+  return WX_OK;
+}
+
+int wx_record_ghosts(void *ctx) {
+  Ctx *c = as_ctx(ctx);
+  int rc = need_state(c);
+  if (rc != WX_OK) return rc;
+  if (!is_fused(c)) return WX_OK;   // the plain form's PRIM keeps its ghost layer
+  rc = fused_ghost_prims(c);
+  if (rc == WX_OK && hipStreamSynchronize(c->st) != hipSuccess) rc = fail(WX_EHIP, "ghost record failed");
+  c->ghosts_recorded = rc == WX_OK;
+  return rc;
+}
+
+int wx_mark_blocks(void *ctx, double refine_thres, double coarse_thres, int *flags, double *maxgrad) {
+  Ctx *c = as_ctx(ctx);
+  int rc = need_state(c);
+  if (rc != WX_OK) return rc;
+  if (!flags) return fail(WX_EINVAL, "null flags");
+  return mark_blocks(c, refine_thres, coarse_thres, flags, maxgrad);
+}
+
+int wx_admesh(void *ctx, double refine_thres, double coarse_thres, int cap, int *nblk_new, int *bids_new) {
+  Ctx *c = as_ctx(ctx);
+  int rc = need_state(c);
+  if (rc != WX_OK) return rc;
+  if (!nblk_new) return fail(WX_EINVAL, "null argument");
+  std::vector<int> flags(c->nblk), out;
+  rc = mark_blocks(c, refine_thres, coarse_thres, flags.data(), nullptr);
+  if (rc != WX_OK) return rc;
+  Numbering N;
+  N.init(c->nroot, c->maxlev);
+  rc = amr_plan(N, c->bids, flags, out);
+  if (rc != WX_OK) return rc;
+  *nblk_new = (int)out.size();
+  if ((int)out.size() > cap || !bids_new)
+    return fail(WX_ENOMEM, "new block list (" + std::to_string(out.size()) + " blocks) exceeds the capacity");
+  if (out == c->bids) {   // no change: the mesh, state and step graph stay
+    std::copy(out.begin(), out.end(), bids_new);
+    return WX_OK;
+  }
+  rc = check_mesh(c, out);
+  if (rc != WX_OK) return rc;
+  std::copy(out.begin(), out.end(), bids_new);
+  return wx_remesh(ctx, (int)out.size(), out.data());
 }
 
 }  // extern "C"

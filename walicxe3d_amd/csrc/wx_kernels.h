@@ -16,6 +16,8 @@ void plain_upwind(const DevCfg &c, const double *U, const double *P, const doubl
 void plain_viscous(const DevCfg &c, const double *UP, double *U, hipStream_t st);
 void plain_timestep(const DevCfg &c, const double *P, unsigned long long *dtbits, hipStream_t st);
 void plain_cooling(const DevCfg &c, double *U, double *P, double dt, hipStream_t st);
+// per-block largest relative pressure / density gradient (markByPhysical)
+void plain_mark(const DevCfg &c, const double *P, double *maxgrad, hipStream_t st);
 size_t plain_flux_elems(int nblk, int nv);
 
 }  // namespace wx

@@ -257,6 +257,39 @@ __global__ void __launch_bounds__(256) k_cooling(DevCfg c, double *U, double *P,
 }
 
 // ---------------------------------------------------------------------------
+// markByPhysical (source/amr.f90:142-230), the per-block part: the largest
+// |q(i+1) - q(i-1)| / q(i) / (2 dx) along x, y and z over the interior
+// cells, for q = pressure and q = density, read from PRIM with its depth-1
+// face ghosts.  dx is the level's x spacing in all three directions, as in
+// the reference.  The reference returns at the first cell whose gradient
+// reaches the refinement threshold; which flag it returns then depends only
+// on the level, so the flag is a function of this maximum (set on the host).
+// grid nblk, block 256: lane (i, j) walks k.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_mark(DevCfg c, const double *P, double *maxgrad) {
+  __shared__ double part[4];
+  const int b = blockIdx.x;
+  const int i = (threadIdx.x & (NC - 1)) + 1, j = threadIdx.x / NC + 1;
+  const double two_dx = 2.0 * c.dx[0][c.info[b].level - 1];
+  double m = 0.0;
+  for (int s = 0; s < 2; s++) {
+    const double *A = P + ((size_t)b * c.nv + (s == 0 ? 4 : 0)) * NCELL;
+    for (int k = 1; k <= NC; k++) {
+      const int q = cid(i, j, k);
+      const double a = A[q];
+      const double gx = fabs(A[q + 1] - A[q - 1]) / a / two_dx;
+      const double gy = fabs(A[q + NB] - A[q - NB]) / a / two_dx;
+      const double gz = fabs(A[q + NB * NB] - A[q - NB * NB]) / a / two_dx;
+      m = fmax(m, fmax(gx, fmax(gy, gz)));
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) maxgrad[b] = fmax(fmax(part[0], part[1]), fmax(part[2], part[3]));
+}
+
+// ---------------------------------------------------------------------------
 // host launchers
 // ---------------------------------------------------------------------------
 void plain_boundary(const DevCfg &c, double *A, int depth, hipStream_t st) {
@@ -282,6 +315,9 @@ void plain_timestep(const DevCfg &c, const double *P, unsigned long long *dtbits
 }
 void plain_cooling(const DevCfg &c, double *U, double *P, double dt, hipStream_t st) {
   hipLaunchKernelGGL(k_cooling, dim3(16, c.nblk), dim3(256), 0, st, c, U, P, dt);
+}
+void plain_mark(const DevCfg &c, const double *P, double *maxgrad, hipStream_t st) {
+  hipLaunchKernelGGL(k_mark, dim3(c.nblk), dim3(256), 0, st, c, P, maxgrad);
 }
 size_t plain_flux_elems(int nblk, int nv) { return (size_t)nblk * 3 * nv * NIF; }
 

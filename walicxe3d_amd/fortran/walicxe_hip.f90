@@ -19,6 +19,9 @@ module walicxe_hip
   integer(c_int), parameter :: WX_CELLS_ALL = 0
   integer(c_int), parameter :: WX_CELLS_PHYS = 1
   integer(c_int), parameter :: WX_CELLS_GHOST = 2
+  integer(c_int), parameter :: WX_FLAG_NONE = 0
+  integer(c_int), parameter :: WX_FLAG_REFINE = 1
+  integer(c_int), parameter :: WX_FLAG_COARSE = 2
 
   ! struct wx_config
   type, bind(C) :: wx_config
@@ -66,6 +69,40 @@ module walicxe_hip
       type(c_ptr), value :: ctx
       integer(c_int), value :: nblk
       integer(c_int), intent(in) :: bids(*)
+    end function
+
+    integer(c_int) function wx_mark_blocks(ctx, refine_thres, coarse_thres, flags, maxgrad) &
+        bind(C, name="wx_mark_blocks")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx, maxgrad
+      real(c_double), value :: refine_thres, coarse_thres
+      integer(c_int), intent(out) :: flags(*)
+    end function
+
+    integer(c_int) function wx_record_ghosts(ctx) bind(C, name="wx_record_ghosts")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+
+    integer(c_int) function wx_admesh_plan(nroot, maxlev, nblk, bids, flags, flags_out, cap, nblk_new, bids_new) &
+        bind(C, name="wx_admesh_plan")
+      import :: c_int, c_ptr
+      integer(c_int), intent(in) :: nroot(3)
+      integer(c_int), value :: maxlev, nblk, cap
+      integer(c_int), intent(in) :: bids(*), flags(*)
+      type(c_ptr), value :: flags_out
+      integer(c_int), intent(out) :: nblk_new
+      integer(c_int), intent(out) :: bids_new(*)
+    end function
+
+    integer(c_int) function wx_admesh(ctx, refine_thres, coarse_thres, cap, nblk_new, bids_new) &
+        bind(C, name="wx_admesh")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: refine_thres, coarse_thres
+      integer(c_int), value :: cap
+      integer(c_int), intent(out) :: nblk_new
+      integer(c_int), intent(out) :: bids_new(*)
     end function
 
     integer(c_int) function wx_upload_state(ctx, u, layout, nbmax, local_blocks) bind(C, name="wx_upload_state")

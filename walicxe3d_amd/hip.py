@@ -66,7 +66,17 @@ SIGNATURES = {
     "wx_last_timing": (C.c_int, [C.c_void_p, _dp, _dp, _dp]),
     "wx_time_stages": (C.c_int, [C.c_void_p, C.c_int]),
     "wx_remesh": (C.c_int, [C.c_void_p, C.c_int, _ip]),
+    "wx_mark_blocks": (C.c_int, [C.c_void_p, C.c_double, C.c_double, _ip, _dp]),
+    "wx_admesh_plan": (C.c_int, [_ip, C.c_int, C.c_int, _ip, _ip, _ip, C.c_int, _ip, _ip]),
+    "wx_admesh": (C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_int, _ip, _ip]),
+    "wx_record_ghosts": (C.c_int, [C.c_void_p]),
 }
+
+FLAG_NONE, FLAG_REFINE, FLAG_COARSE = 0, 1, 2
+
+# refineThres / coarseThres of the reference's parameters (source/parameters.f90:119-120)
+REFINE_THRES = 0.3
+COARSE_THRES = 0.03
 
 _lib = None
 
@@ -98,6 +108,23 @@ def _check(rc, what):
 
 def _p(a):
     return a.ctypes.data_as(_dp)
+
+
+def admesh_plan(mesh, bids, flags):
+    """checkProximity + refine / coarsen list of admesh for given flags
+    (wx_admesh_plan): returns (new bids, flags after the proximity rule)."""
+    b = np.ascontiguousarray(bids, dtype=np.int32)
+    f = np.ascontiguousarray(flags, dtype=np.int32)
+    assert b.shape == f.shape and b.ndim == 1
+    fo = np.empty_like(f)
+    cap = 8 * len(b)
+    out = np.empty(cap, dtype=np.int32)
+    n = C.c_int(0)
+    nroot = (C.c_int * 3)(*mesh.nroot)
+    _check(load().wx_admesh_plan(nroot, mesh.maxlev, len(b), b.ctypes.data_as(_ip), f.ctypes.data_as(_ip),
+                                 fo.ctypes.data_as(_ip), cap, C.byref(n), out.ctypes.data_as(_ip)),
+           "wx_admesh_plan")
+    return out[:n.value].copy(), fo
 
 
 def make_config(w, form=PLAIN_FP64, table=None, winds=None):
@@ -172,6 +199,31 @@ class Walicxe3D:
         self.bids = b
         self.nblk = len(b)
         return rc
+
+    def record_ghosts(self):
+        """Fused form: keep the ghost layer of the current state for the next
+        markByPhysical / admesh (call before the step they follow)."""
+        return _check(load().wx_record_ghosts(self._h), "wx_record_ghosts")
+
+    def markByPhysical(self, refine_thres=REFINE_THRES, coarse_thres=COARSE_THRES):
+        """Flags and largest relative gradient of every block (wx_mark_blocks)."""
+        f = np.empty(self.nblk, dtype=np.int32)
+        g = np.empty(self.nblk)
+        _check(load().wx_mark_blocks(self._h, refine_thres, coarse_thres, f.ctypes.data_as(_ip), _p(g)),
+               "wx_mark_blocks")
+        return f, g
+
+    def admesh(self, refine_thres=REFINE_THRES, coarse_thres=COARSE_THRES, cap=None):
+        """The reference's admesh on the device state (wx_admesh); returns
+        the new block list (also kept in self.bids)."""
+        cap = cap if cap is not None else 8 * self.nblk
+        out = np.empty(cap, dtype=np.int32)
+        n = C.c_int(0)
+        rc = _check(load().wx_admesh(self._h, refine_thres, coarse_thres, cap, C.byref(n),
+                                     out.ctypes.data_as(_ip)), "wx_admesh")
+        self.bids = out[:n.value].copy()
+        self.nblk = n.value
+        return self.bids
 
     def close(self):
         if getattr(self, "_h", None):

@@ -307,6 +307,15 @@ _reg(_w(name="amr_small", mhd=False, solver="hllc", limiter="woodward", gamma_ex
         nroot=(1, 1, 1), maxlev=4, base_level=2, phys=(20.0 * PC, 20.0 * PC, 20.0 * PC), ic="snr", seed=8,
         nsteps=8, cooling=True, refine_rule="snr", **_PHYS_UNITS,
         description="small SNR on 3 AMR levels with radiative cooling"))
+# Mesh adaptation coverage (admesh every step): an off-centre blast in a
+# near-uniform medium, the initial refinement centred elsewhere, so that the
+# first updates refine around the blast, force coarser neighbours by the
+# proximity rule and coarsen families in the quiet region
+_reg(_w(name="amr_adapt", mhd=False, solver="hllc", limiter="minmod", gamma_expr="5.0/3.0",
+        gamma=5.0 / 3.0, cfl=0.4, visc_eta=5.0e-3, eight_wave=False, bc=FREE6,
+        nroot=(1, 1, 1), maxlev=5, base_level=2, phys=(20.0 * PC, 20.0 * PC, 20.0 * PC), ic="blast_amr",
+        seed=9, nsteps=8, cooling=True, refine_rule="snr", **_PHYS_UNITS,
+        description="off-centre blast in a near-uniform medium on 3 AMR levels, cooling, mesh adapted every step"))
 # C4 reduced (oracle): root 4^3, 4 levels
 _reg(_w(name="snr_amr4", mhd=False, solver="hllc", limiter="minmod", gamma_expr="5.0/3.0",
         gamma=5.0 / 3.0, cfl=0.4, visc_eta=5.0e-3, eight_wave=False, bc=FREE6,
@@ -522,18 +531,22 @@ def block_ic(w: Workload, m: Mesh, bid: int) -> np.ndarray:
         bx = -np.sin(2.0 * pi * Y) * b0
         by = np.sin(4.0 * pi * X) * b0
         return _prim_to_cons(w, rho, vx, vy, vz, p, bx, by, zeros)
-    if w.ic in ("snr", "winds"):
+    if w.ic in ("snr", "winds", "blast_amr"):
         # ambient n_H = 1 cm^-3 at T = 1e4 K with lognormal clumps / +-5% noise
+        # (+-0.01% for blast_amr)
         if w.ic == "snr":
             G = _smooth_gaussian(w, X, Y, Z)
             rho = np.exp(0.5 * G - 0.125)
-        else:
+        elif w.ic == "winds":
             rho = 1.0 + 0.05 * uniform_pm1(w.seed, 1, l, g)
+        else:
+            rho = 1.0 + 1.0e-4 * uniform_pm1(w.seed, 1, l, g)
         T = 1.0e4
         p = rho * w.d_sc / (w.mu0 * AMU) * KB * T / w.p_sc
-        if w.ic == "snr":
+        if w.ic in ("snr", "blast_amr"):
             L = w.phys[0] / w.l_sc
-            r = np.sqrt((X - 0.5 * L) ** 2 + (Y - 0.5 * L) ** 2 + (Z - 0.5 * L) ** 2)
+            c = (0.5, 0.5, 0.5) if w.ic == "snr" else (0.40, 0.52, 0.58)
+            r = np.sqrt((X - c[0] * L) ** 2 + (Y - c[1] * L) ** 2 + (Z - c[2] * L) ** 2)
             rsn = 1.0 * PC / w.l_sc
             vol = 4.0 / 3.0 * math.pi * rsn ** 3
             esn = 1.0e51 / (w.p_sc * w.l_sc ** 3)
