@@ -148,7 +148,9 @@ int wx_copy_raw(void *ctx, int which, double *out);
    reference does: kept by copied blocks, injected into refined children,
    recomputed from U in new coarse blocks; the fused form recomputes its
    state from U.  Returns WX_ETOPO, with the mesh and state untouched, when a
-   new block would need a restriction over more than one level. */
+   new block would need a restriction over more than one level or the new
+   list is not a mesh wx_set_mesh accepts (a jump of two levels across a
+   face, a periodic face without a partner). */
 int wx_remesh(void *ctx, int nblk, const int *bids);
 
 /* Refinement flags (WX_FLAG_*) of admesh */

@@ -53,6 +53,12 @@ static int run_case(const char *path, int nsteps) {
   if (nb2 == -1) {
     std::vector<int> nb(8 * 4096);
     for (int n = 0; n < nsteps && rc >= 0; n++) {
+      // fused form: the recorded ghost layer on even steps, the current
+      // state's exchange on odd ones
+      if (cfg.form != WX_FORM_PLAIN_FP64 && n % 2 == 0) {
+        rc = wx_record_ghosts(ctx);
+        if (rc < 0) { printf("record %d %s\n", rc, wx_last_error()); return 1; }
+      }
       if (cfg.form == WX_FORM_PLAIN_FP64) rc = wx_step(ctx, &it, &dt, &time, &nextout, dtout_tsc[0], dtout_tsc[1]);
       else rc = wx_run(ctx, 1, &it, &dt, &time, &nextout, dtout_tsc[0], dtout_tsc[1]);
       if (rc < 0) break;

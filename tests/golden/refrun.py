@@ -94,8 +94,8 @@ def run(w, nsteps, dumps, workdir=None, unit_op=None, unit_in=None, timeout=3600
     write_ic_with_passive(os.path.join(wd, "ic.bin"), w, blocks, U)
     dl = ",".join(str(d) for d in dumps)
     with open(os.path.join(wd, "harness.nml"), "w") as f:
-        f.write("&ctl\n mode='run'\n nsteps=%d\n ndump=%d\n dumps=%s\n tstop=%r\n amr=%s\n/\n"
-                % (nsteps, len(dumps), dl or "-1", float(tstop), ".true." if amr else ".false."))
+        f.write("&ctl\n mode='run'\n nsteps=%d\n ndump=%d\n dumps=%s\n tstop=%r\n%s/\n"
+                % (nsteps, len(dumps), dl or "-1", float(tstop), " amr=.true.\n" if amr else ""))
     cmd = [exe] if nprocs == 1 else [MPIEXEC, "-n", str(nprocs), exe]
     r = subprocess.run(cmd, cwd=wd, timeout=timeout, capture_output=True, text=True)
     if r.returncode != 0:

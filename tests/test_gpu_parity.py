@@ -310,8 +310,9 @@ def test_fused_remesh_recaptures_graph():
 def test_device_remesh_matches_host_regrid(form):
     """wx_remesh carries the state to a new block list on the device (copy,
     octant injection, 2x2x2 restriction); it equals the host regrid of the
-    downloaded state bit for bit, in both directions, and the next steps
-    equal those of a context given that state by upload."""
+    downloaded state bit for bit, in both directions (the plain form's PRIM
+    too: it is carried, not recomputed), and the next steps equal those of a
+    context given that state (U, and PRIM in the plain form) by upload."""
     from walicxe3d_amd import hip
     from walicxe3d_amd import regrid
     f = hip.FUSED_FP32 if form == "fused_fp32" else hip.PLAIN_FP64
@@ -325,16 +326,21 @@ def test_device_remesh_matches_host_regrid(form):
     g.upload(ic_in_order(w1, b1))
     for _ in range(2):
         g.step()
-    u1 = g.download(prim=False)
+    u1, p1 = g.download()
     g.remesh(b2)                               # refine: copies and injections
-    u2 = g.download(prim=False)
+    u2, p2 = g.download()
     assert np.array_equal(u2, regrid.remap(m, b1, u1, b2))
+    if f == hip.PLAIN_FP64:
+        # PRIM is carried as U is (the cooled pressure, not the one U gives)
+        assert np.array_equal(p2, regrid.remap(m, b1, p1, b2))
     g.remesh(b1)                               # coarsen back: copies and restrictions
     u3 = g.download(prim=False)
     assert np.array_equal(u3, regrid.remap(m, b2, u2, b1))
     g.remesh(b2)
     h = gpu(w2, f, b2)
     h.upload(regrid.remap(m, b1, u3, b2))
+    if f == hip.PLAIN_FP64:
+        h.upload_prims(g.download()[1])
     h.it.value, h.dt.value, h.time.value, h.nextout.value = g.it.value, g.dt.value, g.time.value, g.nextout.value
     for _ in range(2):
         g.step()

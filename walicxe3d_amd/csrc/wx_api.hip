@@ -52,7 +52,8 @@ struct Ctx {
   double *dU = nullptr, *dUP = nullptr, *dP = nullptr, *dF = nullptr, *dTmp = nullptr;
   unsigned long long *d_scal = nullptr;  // [0] dt bits, [1] floor count
   bool have_state = false;
-  bool ghosts_recorded = false;      // fused form: PRIM ghost layer of wx_record_ghosts in dP
+  bool ghosts_recorded = false;      // fused form: ghost layer of wx_record_ghosts in d_ghosts
+  double *d_ghosts = nullptr;        // [blk][rho, p][6][256]
   FusedState fused;
 };
 
@@ -66,6 +67,8 @@ void free_mesh(Ctx *c) {
   c->d_info = nullptr;
   c->dU = c->dUP = c->dP = c->dF = c->dTmp = nullptr;
   fused_free(c->fused);
+  hipFree(c->d_ghosts);
+  c->d_ghosts = nullptr;
   c->have_state = false;
   c->ghosts_recorded = false;
 }
@@ -323,7 +326,7 @@ bool is_fused(const Ctx *c) { return c->cfg.form != WX_FORM_PLAIN_FP64; }
 
 int need_state(Ctx *c) {
   if (!c) return fail(WX_EINVAL, "null context");
-  if (!c->dU) return fail(WX_ESTATE, "no mesh: call wx_set_mesh first");
+  if (!c->d_info) return fail(WX_ESTATE, "no mesh: call wx_set_mesh first");
   if (!c->have_state) return fail(WX_ESTATE, "no state: call wx_upload_state first");
   return WX_OK;
 }
@@ -585,31 +588,17 @@ static int check_mesh(const Ctx *c, const std::vector<int> &bids) {
   return build_topology(&t, bids.data());
 }
 
-// The fused form keeps no ghost layers: the depth-1 exchange of its current
-// state into the plain arrays, PRIM on the face ghosts (dP's ghost layer).
-static int fused_ghost_prims(Ctx *c) {
-  if (fused_download(c->fused, c->dc, c->dU, c->dP, c->st)) return fail(WX_EHIP, "fused state read-back failed");
-  plain_boundary(c->dc, c->dU, 1, c->st);
-  plain_prims(c->dc, c->dU, c->dP, WX_CELLS_GHOST, nullptr, c->st);
-  return launch_check("ghost prims");
-}
-
-// markByPhysical flags of every block of the context
+// markByPhysical flags of every block of the context.  The fused form keeps
+// no ghost layers: its kernel forms the depth-1 ghosts of the current state
+// (or reads the layer wx_record_ghosts kept).
 static int mark_blocks(Ctx *c, double refine_thres, double coarse_thres, int *flags, double *maxgrad) {
-  if (is_fused(c)) {
-    if (c->ghosts_recorded) {
-      // interior PRIM of the current state; the ghost layer stays the recorded one
-      if (fused_download(c->fused, c->dc, c->dU, c->dP, c->st))
-        return fail(WX_EHIP, "fused state read-back failed");
-    } else {
-      int rc = fused_ghost_prims(c);
-      if (rc != WX_OK) return rc;
-    }
-  }
   std::vector<double> m(c->nblk);
   double *d_m = nullptr;
   HIPCHK(hipMalloc(&d_m, m.size() * sizeof(double)));
-  plain_mark(c->dc, c->dP, d_m, c->st);
+  if (is_fused(c))
+    fused_mark(c->dc, c->fused.U, c->ghosts_recorded ? c->d_ghosts : nullptr, d_m, c->st);
+  else
+    plain_mark(c->dc, c->dP, d_m, c->st);
   int rc = launch_check("mark");
   if (rc == WX_OK && hipMemcpyAsync(m.data(), d_m, m.size() * sizeof(double), hipMemcpyDeviceToHost, c->st) != hipSuccess)
     rc = fail(WX_EHIP, "mark read-back failed");
@@ -712,10 +701,12 @@ int wx_set_mesh(void *ctx, const int nroot[3], int maxlev, const double *dxyz, i
   size_t n = (size_t)nblk * c->cfg.nv * NCELL;
   HIPCHK(hipMalloc(&c->d_info, sizeof(BlockInfo) * nblk));
   HIPCHK(hipMemcpy(c->d_info, c->info.data(), sizeof(BlockInfo) * nblk, hipMemcpyHostToDevice));
-  HIPCHK(hipMalloc(&c->dU, n * sizeof(double)));
-  HIPCHK(hipMalloc(&c->dP, n * sizeof(double)));
-  HIPCHK(hipMemset(c->dP, 0, n * sizeof(double)));
   if (!is_fused(c)) {
+    // the plain form's arrays with ghost layers; the fused form keeps its
+    // state without ghosts (FusedState) and needs none of them
+    HIPCHK(hipMalloc(&c->dU, n * sizeof(double)));
+    HIPCHK(hipMalloc(&c->dP, n * sizeof(double)));
+    HIPCHK(hipMemset(c->dP, 0, n * sizeof(double)));
     HIPCHK(hipMalloc(&c->dUP, n * sizeof(double)));
     HIPCHK(hipMemset(c->dUP, 0, n * sizeof(double)));
     HIPCHK(hipMalloc(&c->dF, plain_flux_elems(nblk, c->cfg.nv) * sizeof(double)));
@@ -730,18 +721,24 @@ int wx_set_mesh(void *ctx, const int nroot[3], int maxlev, const double *dxyz, i
 }
 
 // Interior state [nblk][nv][16^3] already on the device into the context:
-// ghosted U with zero ghosts, then PRIM (plain form) or the fused state and
-// its first dt candidate.
-static int upload_device(Ctx *c, const double *d_in) {
+// the fused state and its first dt candidate (copied, or with adopt the
+// buffer itself, which the context then owns), or the plain form's ghosted
+// U with zero ghosts and its PRIM.
+static int upload_device(Ctx *c, double *d_in, bool adopt = false) {
   int nv = c->cfg.nv;
+  c->ghosts_recorded = false;
+  if (is_fused(c)) {
+    if (fused_upload(c->fused, c->dc, d_in, adopt, c->st)) return fail(WX_EHIP, fused_error());
+    c->have_state = true;
+    return WX_OK;
+  }
   size_t total = (size_t)c->nblk * nv * NCELL;
   hipLaunchKernelGGL(k_scatter, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, c->st, nv, c->nblk, d_in, c->dU, 1);
   int rc = launch_check("upload");
   HIPCHK(hipStreamSynchronize(c->st));
+  if (adopt) hipFree(d_in);
   if (rc != WX_OK) return rc;
   c->have_state = true;
-  c->ghosts_recorded = false;
-  if (is_fused(c)) return fused_upload(c->fused, c->dc, c->dU, c->st) ? WX_EHIP : WX_OK;
   long bad = 0;
   rc = update_prims(c, &bad);
   if (rc != WX_OK) return rc;
@@ -751,7 +748,7 @@ static int upload_device(Ctx *c, const double *d_in) {
 int wx_upload_state(void *ctx, const double *u, int layout, int nbmax, const int *lb) {
   Ctx *c = as_ctx(ctx);
   if (!c || !u) return fail(WX_EINVAL, "null argument");
-  if (!c->dU) return fail(WX_ESTATE, "no mesh");
+  if (!c->d_info) return fail(WX_ESTATE, "no mesh");
   int nv = c->cfg.nv;
   size_t ni = (size_t)c->nblk * nv * NINT;
   std::vector<double> tmp;
@@ -807,19 +804,25 @@ int wx_download_state(void *ctx, double *u, double *prim, int layout, int nbmax,
   if (rc != WX_OK) return rc;
   int nv = c->cfg.nv;
   size_t ni = (size_t)c->nblk * nv * NINT;
-  if (is_fused(c)) {
-    if (fused_download(c->fused, c->dc, c->dU, c->dP, c->st)) return fail(WX_EHIP, "fused download failed");
-  }
   double *d_out = nullptr;
   HIPCHK(hipMalloc(&d_out, ni * sizeof(double)));
   std::vector<double> tmp(layout == WX_LAYOUT_FORTRAN ? ni : 0);
   for (int w = 0; w < 2; w++) {
     double *dst = w == 0 ? u : prim;
     if (!dst) continue;
-    hipLaunchKernelGGL(k_gather, dim3((unsigned)((ni + 255) / 256)), dim3(256), 0, c->st, nv, c->nblk,
-                       w == 0 ? c->dU : c->dP, d_out);
+    const double *from = d_out;
+    if (is_fused(c)) {
+      // the fused state is already in the block layout of the interface
+      if (w == 0)
+        from = c->fused.U;
+      else if (fused_prims(c->fused, c->dc, d_out, c->st))
+        return fail(WX_EHIP, fused_error());
+    } else {
+      hipLaunchKernelGGL(k_gather, dim3((unsigned)((ni + 255) / 256)), dim3(256), 0, c->st, nv, c->nblk,
+                         w == 0 ? c->dU : c->dP, d_out);
+    }
     double *h = layout == WX_LAYOUT_FORTRAN ? tmp.data() : dst;
-    HIPCHK(hipMemcpyAsync(h, d_out, ni * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipMemcpyAsync(h, from, ni * sizeof(double), hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
     if (layout == WX_LAYOUT_FORTRAN) to_fortran_or_back(c, tmp.data(), dst, false, nbmax, lb);
   }
@@ -995,15 +998,37 @@ int wx_remesh(void *ctx, int nblk, const int *bids) {
   if (!bids || nblk < 1) return fail(WX_EINVAL, "bad block list");
   std::vector<RemapSrc> table;
   rc = remap_table(c, nblk, bids, table);
+  if (rc == WX_OK) rc = check_mesh(c, std::vector<int>(bids, bids + nblk));
   if (rc != WX_OK) return rc;              // the current mesh and state are untouched
   const int nv = c->cfg.nv;
+  HIPCHK(hipStreamSynchronize(c->st));
+  size_t nold = (size_t)c->nblk * nv * NINT, nnew = (size_t)nblk * nv * NINT;
+  if (is_fused(c)) {
+    // the fused state has the block layout k_remap reads and writes: the new
+    // state is remapped straight from it and becomes the new fused state
+    double *d_new = nullptr;
+    RemapSrc *d_map = nullptr;
+    HIPCHK(hipMalloc(&d_new, nnew * sizeof(double)));
+    HIPCHK(hipMalloc(&d_map, table.size() * sizeof(RemapSrc)));
+    HIPCHK(hipMemcpyAsync(d_map, table.data(), table.size() * sizeof(RemapSrc), hipMemcpyHostToDevice, c->st));
+    hipLaunchKernelGGL(k_remap, dim3(nblk, nv), dim3(256), 0, c->st, nv, d_map, c->fused.U, d_new);
+    rc = launch_check("remesh");
+    HIPCHK(hipStreamSynchronize(c->st));
+    hipFree(d_map);
+    if (rc == WX_OK) {
+      const std::vector<double> dxyz = c->dxyz;
+      int nroot[3] = {c->nroot[0], c->nroot[1], c->nroot[2]};
+      rc = wx_set_mesh(ctx, nroot, c->maxlev, dxyz.data(), nblk, bids);
+    }
+    if (rc != WX_OK) {
+      hipFree(d_new);
+      return rc;
+    }
+    return upload_device(c, d_new, true);
+  }
   // the plain form also carries PRIM (its interior pressure is the one
   // cooling left, not the one U gives back)
-  const bool carry = !is_fused(c);
-  HIPCHK(hipStreamSynchronize(c->st));
-  if (is_fused(c) && fused_download(c->fused, c->dc, c->dU, c->dP, c->st))
-    return fail(WX_EHIP, "fused state read-back failed");
-  size_t nold = (size_t)c->nblk * nv * NINT, nnew = (size_t)nblk * nv * NINT;
+  const bool carry = true;
   double *d_old = nullptr, *d_new = nullptr, *d_newp = nullptr;
   RemapSrc *d_map = nullptr;
   HIPCHK(hipMalloc(&d_old, nold * sizeof(double)));
@@ -1060,7 +1085,9 @@ int wx_record_ghosts(void *ctx) {
   int rc = need_state(c);
   if (rc != WX_OK) return rc;
   if (!is_fused(c)) return WX_OK;   // the plain form's PRIM keeps its ghost layer
-  rc = fused_ghost_prims(c);
+  if (!c->d_ghosts) HIPCHK(hipMalloc(&c->d_ghosts, (size_t)c->nblk * 2 * 6 * 256 * sizeof(double)));
+  fused_ghost_prims(c->dc, c->fused.U, c->d_ghosts, c->st);
+  rc = launch_check("ghost record");
   if (rc == WX_OK && hipStreamSynchronize(c->st) != hipSuccess) rc = fail(WX_EHIP, "ghost record failed");
   c->ghosts_recorded = rc == WX_OK;
   return rc;

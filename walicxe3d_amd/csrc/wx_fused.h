@@ -45,8 +45,11 @@ struct FusedState {
 
 int fused_alloc(FusedState &f, const DevCfg &c, const std::vector<BlockInfo> &info, hipStream_t st);
 void fused_free(FusedState &f);
-int fused_upload(FusedState &f, const DevCfg &c, const double *dU_ghosted, hipStream_t st);
-int fused_download(FusedState &f, const DevCfg &c, double *dU_ghosted, double *dP_ghosted, hipStream_t st);
+// State in from U ([blk][nv][16^3] FP64, device): copied, or with adopt the
+// buffer itself becomes the state (the library frees it); dt re-initialised.
+int fused_upload(FusedState &f, const DevCfg &c, double *U, bool adopt, hipStream_t st);
+// PRIM of the state into P ([blk][nv][16^3] FP64, device)
+int fused_prims(FusedState &f, const DevCfg &c, double *P, hipStream_t st);
 int fused_timestep(FusedState &f, const DevCfg &c, hipStream_t st, double *dt_raw);
 int fused_run(FusedState &f, const DevCfg &c, hipStream_t st, int nsteps, int *it, double *dt, double *time,
               int *nextout, double dtout, double t_sc);

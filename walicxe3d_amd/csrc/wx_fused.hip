@@ -72,28 +72,17 @@ __global__ void k_step_post(StepState *s) {
   if (s->dump) s->nextout += 1;
 }
 
-// conversions between the ghosted FP64 layout and the compact one
-__global__ void k_compact(int nv, int nblk, const double *src, double *dst) {
-  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (size_t)nblk * nv * NINT) return;
-  int q = (int)(t % NINT);
-  size_t bv = t / NINT;
-  dst[t] = src[bv * NCELL + cid(q % NC + 1, (q / NC) % NC + 1, q / (NC * NC) + 1)];
-}
+// PRIM of the state in the same layout ([blk][nv][16^3]), for downloads
 template <int NV>
-__global__ void k_expand(const DevCfg c, int nblk, const double *src, double *dstU, double *dstP) {
+__global__ void k_prims_int(const DevCfg c, int nblk, const double *src, double *dstP) {
   size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (size_t)nblk * NINT) return;
   int q = (int)(t % NINT);
   size_t b = t / NINT;
-  int cq = cid(q % NC + 1, (q / NC) % NC + 1, q / (NC * NC) + 1);
   double u[MAXV], p[MAXV];
   for (int v = 0; v < NV; v++) u[v] = src[(b * NV + v) * NINT + q];
   flow2prim(c.pd, u, p);
-  for (int v = 0; v < NV; v++) {
-    dstU[(b * NV + v) * NCELL + cq] = u[v];
-    dstP[(b * NV + v) * NCELL + cq] = p[v];
-  }
+  for (int v = 0; v < NV; v++) dstP[(b * NV + v) * NINT + q] = p[v];
 }
 
 // ---------------------------------------------------------------------------
@@ -201,9 +190,14 @@ static void launch_dt_init(FusedState &f, const DevCfg &c, hipStream_t st) {
   else hipLaunchKernelGGL(k_dt_init<5>, dim3(f.nblk), dim3(256), 0, st, a);
 }
 
-int fused_upload(FusedState &f, const DevCfg &c, const double *dU, hipStream_t st) {
-  size_t n = (size_t)f.nblk * f.nv * NINT;
-  hipLaunchKernelGGL(k_compact, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, f.nv, f.nblk, dU, f.U);
+int fused_upload(FusedState &f, const DevCfg &c, double *U, bool adopt, hipStream_t st) {
+  if (adopt) {
+    FCHK(hipStreamSynchronize(st));
+    hipFree(f.U);
+    f.U = U;
+  } else {
+    FCHK(hipMemcpyAsync(f.U, U, (size_t)f.nblk * f.nv * NINT * sizeof(double), hipMemcpyDeviceToDevice, st));
+  }
   StepState s{};
   s.dtmin_bits = ~0ull;
   FCHK(hipMemcpyAsync(f.step, &s, sizeof(s), hipMemcpyHostToDevice, st));
@@ -213,12 +207,12 @@ int fused_upload(FusedState &f, const DevCfg &c, const double *dU, hipStream_t s
   return 0;
 }
 
-int fused_download(FusedState &f, const DevCfg &c, double *dU, double *dP, hipStream_t st) {
+int fused_prims(FusedState &f, const DevCfg &c, double *P, hipStream_t st) {
   size_t n = (size_t)f.nblk * NINT;
   if (f.nv == 8)
-    hipLaunchKernelGGL(k_expand<8>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, c, f.nblk, f.U, dU, dP);
+    hipLaunchKernelGGL(k_prims_int<8>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, c, f.nblk, f.U, P);
   else
-    hipLaunchKernelGGL(k_expand<5>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, c, f.nblk, f.U, dU, dP);
+    hipLaunchKernelGGL(k_prims_int<5>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, c, f.nblk, f.U, P);
   FCHK(hipGetLastError());
   FCHK(hipStreamSynchronize(st));
   return 0;

@@ -18,6 +18,10 @@ void plain_timestep(const DevCfg &c, const double *P, unsigned long long *dtbits
 void plain_cooling(const DevCfg &c, double *U, double *P, double dt, hipStream_t st);
 // per-block largest relative pressure / density gradient (markByPhysical)
 void plain_mark(const DevCfg &c, const double *P, double *maxgrad, hipStream_t st);
+// the same on the fused state ([blk][nv][16^3]): recorded ghost layer
+// G[blk][rho, p][6][256], and the per-block maximum (G may be null)
+void fused_ghost_prims(const DevCfg &c, const double *U, double *G, hipStream_t st);
+void fused_mark(const DevCfg &c, const double *U, const double *G, double *maxgrad, hipStream_t st);
 size_t plain_flux_elems(int nblk, int nv);
 
 }  // namespace wx

@@ -290,6 +290,116 @@ __global__ void __launch_bounds__(256) k_mark(DevCfg c, const double *P, double 
 }
 
 // ---------------------------------------------------------------------------
+// The same for the fused form, whose state has no ghost layers
+// ([blk][nv][16^3] FP64): density and pressure of the depth-1 face ghosts
+// are formed from the sources the exchange reads (same-level copy,
+// coarse->fine injection, fine->coarse 8-cell average, domain faces) and
+// the wind sources, with the plain form's arithmetic, then flow2prim.
+// ---------------------------------------------------------------------------
+__device__ inline double ghost_value_int(const double *U, int nv, const GhostSrc &s, int v) {
+  const double *S = U + ((size_t)s.blk * nv + v) * NINT;
+  if (s.kind == 1) {
+    double acc = 0.0;
+    for (int dx = 0; dx < 2; dx++)
+      for (int dy = 0; dy < 2; dy++)
+        for (int dz = 0; dz < 2; dz++)
+          acc = acc + S[((s.c[2] + dz - 1) * NC + (s.c[1] + dy - 1)) * NC + (s.c[0] + dx - 1)] / 8.0;
+    return acc;
+  }
+  double val = S[((s.c[2] - 1) * NC + (s.c[1] - 1)) * NC + (s.c[0] - 1)];
+  if (s.kind == 2 && v == s.flip) val = -1.0 * val;
+  return val;
+}
+
+// density and pressure of ghost (a1, a2) of face f, layer 1, of block b
+__device__ inline void ghost_rho_p(const DevCfg &c, const double *U, int b, int f, int a1, int a2, double &rho,
+                                   double &pres) {
+  GhostSrc s;
+  double u[MAXV], p[MAXV];
+  if (!ghost_source(c.info, c.bc, b, f, 1, a1, a2, s)) {
+    rho = pres = 0.0;
+    return;
+  }
+  for (int v = 0; v < c.nv; v++) u[v] = ghost_value_int(U, c.nv, s, v);
+  int n, t1, t2, d[3];
+  face_axes(f, n, t1, t2);
+  d[n] = (f & 1) ? NC + 1 : 0;
+  d[t1] = a1;
+  d[t2] = a2;
+  wind_state(c, b, d[0], d[1], d[2], u);
+  flow2prim(c.pd, u, p);
+  rho = p[0];
+  pres = p[4];
+}
+
+// Recorded ghost layer: G[blk][2 (rho, p)][6 faces][256].  grid (6, nblk).
+__global__ void __launch_bounds__(256) k_ghost_prims_fused(DevCfg c, const double *U, double *G) {
+  const int b = blockIdx.y, f = blockIdx.x;
+  double rho, pres;
+  ghost_rho_p(c, U, b, f, threadIdx.x % NC + 1, threadIdx.x / NC + 1, rho, pres);
+  G[(((size_t)b * 2 + 0) * 6 + f) * 256 + threadIdx.x] = rho;
+  G[(((size_t)b * 2 + 1) * 6 + f) * 256 + threadIdx.x] = pres;
+}
+
+// markByPhysical on the fused state: density and pressure of the block and
+// its depth-1 face ghosts (from G, or formed from U when G is null) staged
+// in LDS as an 18^3 cube, then the maximum of k_mark.  grid nblk, 1024
+// threads (93 KB of LDS: one workgroup of 16 waves per CU).
+__global__ void __launch_bounds__(1024) k_mark_fused(DevCfg c, const double *U, const double *G, double *maxgrad) {
+  constexpr int E = NC + 2;
+  __shared__ double sr[E * E * E], sp[E * E * E];
+  __shared__ double part[16];
+  const int b = blockIdx.x, t = threadIdx.x, nv = c.nv;
+  for (int q = t; q < NINT; q += 1024) {
+    double u[MAXV], p[MAXV];
+    for (int v = 0; v < nv; v++) u[v] = U[((size_t)b * nv + v) * NINT + q];
+    flow2prim(c.pd, u, p);
+    const int i = q % NC + 1, j = (q / NC) % NC + 1, k = q / (NC * NC) + 1;
+    sr[(k * E + j) * E + i] = p[0];
+    sp[(k * E + j) * E + i] = p[4];
+  }
+  for (int r = t; r < 6 * 256; r += 1024) {
+    const int f = r >> 8, a1 = (r & 255) % NC + 1, a2 = (r & 255) / NC + 1;
+    double rho, pres;
+    if (G) {
+      rho = G[(((size_t)b * 2 + 0) * 6 + f) * 256 + (r & 255)];
+      pres = G[(((size_t)b * 2 + 1) * 6 + f) * 256 + (r & 255)];
+    } else {
+      ghost_rho_p(c, U, b, f, a1, a2, rho, pres);
+    }
+    int n, t1, t2, d[3];
+    face_axes(f, n, t1, t2);
+    d[n] = (f & 1) ? NC + 1 : 0;
+    d[t1] = a1;
+    d[t2] = a2;
+    sr[(d[2] * E + d[1]) * E + d[0]] = rho;
+    sp[(d[2] * E + d[1]) * E + d[0]] = pres;
+  }
+  __syncthreads();
+  const double two_dx = 2.0 * c.dx[0][c.info[b].level - 1];
+  double m = 0.0;
+  for (int s2 = 0; s2 < 2; s2++) {
+    const double *A = s2 == 0 ? sp : sr;
+    for (int q = t; q < NINT; q += 1024) {
+      const int o = ((q / (NC * NC) + 1) * E + (q / NC) % NC + 1) * E + q % NC + 1;
+      const double a = A[o];
+      const double gx = fabs(A[o + 1] - A[o - 1]) / a / two_dx;
+      const double gy = fabs(A[o + E] - A[o - E]) / a / two_dx;
+      const double gz = fabs(A[o + E * E] - A[o - E * E]) / a / two_dx;
+      m = fmax(m, fmax(gx, fmax(gy, gz)));
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
+  if ((t & 63) == 0) part[t >> 6] = m;
+  __syncthreads();
+  if (t == 0) {
+    double r = part[0];
+    for (int w = 1; w < 16; w++) r = fmax(r, part[w]);
+    maxgrad[b] = r;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // host launchers
 // ---------------------------------------------------------------------------
 void plain_boundary(const DevCfg &c, double *A, int depth, hipStream_t st) {
@@ -318,6 +428,12 @@ void plain_cooling(const DevCfg &c, double *U, double *P, double dt, hipStream_t
 }
 void plain_mark(const DevCfg &c, const double *P, double *maxgrad, hipStream_t st) {
   hipLaunchKernelGGL(k_mark, dim3(c.nblk), dim3(256), 0, st, c, P, maxgrad);
+}
+void fused_ghost_prims(const DevCfg &c, const double *U, double *G, hipStream_t st) {
+  hipLaunchKernelGGL(k_ghost_prims_fused, dim3(6, c.nblk), dim3(256), 0, st, c, U, G);
+}
+void fused_mark(const DevCfg &c, const double *U, const double *G, double *maxgrad, hipStream_t st) {
+  hipLaunchKernelGGL(k_mark_fused, dim3(c.nblk), dim3(1024), 0, st, c, U, G, maxgrad);
 }
 size_t plain_flux_elems(int nblk, int nv) { return (size_t)nblk * 3 * nv * NIF; }
 

@@ -239,6 +239,13 @@ class Walicxe3D:
         assert Ub.shape == (self.nblk, self.w.nv, 16, 16, 16), Ub.shape
         return _check(load().wx_upload_state(self._h, _p(Ub), LAYOUT_BLOCKS, 0, None), "wx_upload_state")
 
+    def upload_prims(self, Pb):
+        """Interior PRIM [nblk][nv][16][16][16] (plain form), e.g. the
+        pressure cooling left, which U does not give back exactly."""
+        Pb = np.ascontiguousarray(Pb, dtype=np.float64)
+        assert Pb.shape == (self.nblk, self.w.nv, 16, 16, 16)
+        return _check(load().wx_upload_prims(self._h, _p(Pb), LAYOUT_BLOCKS, 0, None), "wx_upload_prims")
+
     def download(self, prim=True):
         u = np.empty((self.nblk, self.w.nv, 16, 16, 16))
         p = np.empty_like(u) if prim else None

@@ -7,6 +7,14 @@ launches each stage kernel on its own `reps` times (so counters are
 collected per dispatch) and finally copies a 1 GiB device buffer (a plain
 copy kernel of known size, used to scale the memory counters).
 
+Mesh-adaptation mode:
+    python3 -m walicxe3d_amd.profile_run admesh --workload winds32k --reps 3
+times (wall clock around each synchronous call) wx_mark_blocks with and
+without a recorded ghost layer, wx_admesh_plan, and wx_admesh with a
+coarsening threshold that merges part of the mesh, in the fused form, and
+wx_mark_blocks in the plain form; under `rocprofv3 --kernel-trace --stats`
+the per-kernel times of the same calls.
+
 Summary mode:
     python3 -m walicxe3d_amd.profile_run summarize --fetch DIR --write DIR \
         --workload ot256 --out profiles/stage2_ot256_traffic.txt [--cmd "..."]
@@ -51,6 +59,60 @@ def _run(args):
     torch.cuda.synchronize()
     print("profile_run done: %s, %d stage launches each, %d-byte copy" % (w.name, args.reps, COPY_BYTES),
           flush=True)
+    return 0
+
+
+def _admesh(args):
+    import json
+    import time
+
+    import numpy as np
+
+    from walicxe3d_amd import hip
+    from walicxe3d_amd import workloads as W
+
+    w = W.get(args.workload)
+    m = w.mesh()
+    slots, blocks, U0 = W.initial_state(w)
+    b0 = np.asarray(blocks, dtype=np.int32)
+    out = dict(workload=w.name, blocks=len(b0), reps=args.reps)
+
+    def timed(fn):
+        best = None
+        for _ in range(args.reps):
+            t = time.perf_counter()
+            r = fn()
+            dt = time.perf_counter() - t
+            best = dt if best is None else min(best, dt)
+        return r, 1e3 * best
+
+    g = hip.Walicxe3D(w, form=hip.FUSED_FP32, bids=b0)
+    g.upload(U0)
+    g.run(1)
+    (f, grad), out["fused_mark_exchange_ms"] = timed(lambda: g.markByPhysical())
+    _, out["fused_record_ghosts_ms"] = timed(g.record_ghosts)
+    g.run(1)
+    (f, grad), out["fused_mark_recorded_ms"] = timed(lambda: g.markByPhysical())
+    (plan, _), out["plan_ms"] = timed(lambda: hip.admesh_plan(m, g.bids, f))
+    out["plan_blocks_default_thresholds"] = int(len(plan))
+    # a coarsening threshold at the 40th percentile of the block gradients,
+    # no refinement: part of the families merge
+    ct = float(np.percentile(grad, 40))
+    t = time.perf_counter()
+    new = g.admesh(1e300, ct)
+    out["fused_admesh_ms"] = 1e3 * (time.perf_counter() - t)
+    out["admesh_coarse_thres"] = ct
+    out["admesh_blocks_after"] = int(len(new))
+    t = time.perf_counter()
+    g.run(1)
+    out["fused_first_step_after_ms"] = 1e3 * (time.perf_counter() - t)
+    g.close()
+    p = hip.Walicxe3D(w, form=hip.PLAIN_FP64, bids=b0)
+    p.upload(U0)
+    p.step()
+    _, out["plain_mark_ms"] = timed(lambda: p.markByPhysical())
+    p.close()
+    print("profile_run admesh " + json.dumps(out), flush=True)
     return 0
 
 
@@ -137,6 +199,9 @@ def main(argv=None):
     r = sub.add_parser("run")
     r.add_argument("--workload", default="ot256")
     r.add_argument("--reps", type=int, default=2)
+    a = sub.add_parser("admesh")
+    a.add_argument("--workload", default="winds32k")
+    a.add_argument("--reps", type=int, default=3)
     s = sub.add_parser("summarize")
     s.add_argument("--fetch", required=True)
     s.add_argument("--write", required=True)
@@ -144,6 +209,8 @@ def main(argv=None):
     s.add_argument("--out")
     s.add_argument("--cmd")
     args = ap.parse_args(argv)
+    if args.mode == "admesh":
+        return _admesh(args)
     return _run(args) if args.mode == "run" else _summarize(args)
 
 
