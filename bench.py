@@ -127,15 +127,16 @@ def main():
     # ---- fused FP32 form ----------------------------------------------------
     f = hip.Walicxe3D(w, form=hip.FUSED_FP32, bids=bids)
     f.upload(U0)
-    best = {"grid": 0, "xcd_map": 0}
+    best = {"grid": 0, "xcd_map": 0, "curve": 0}
     if args.no_tune:
         f.set_tuning(0, best["grid"])
         f.set_tuning(1, best["xcd_map"])
+        f.set_tuning(2, best["curve"])
     else:
         from walicxe3d_amd import autotune
         best, table = autotune.tune(f, steps=3)
-        log("autotune: %s" % ", ".join("grid %d xcd %d %.3f ms" % (r["grid"], r["xcd_map"], r["ms"])
-                                       for r in table))
+        log("autotune: %s" % ", ".join("grid %d xcd %d curve %d %.3f ms" % (r["grid"], r["xcd_map"], r["curve"],
+                                                                            r["ms"]) for r in table))
     f.run(args.warmup)
     remesh = None
     if args.remesh_every > 0:
@@ -246,7 +247,8 @@ def main():
         "config": {"workload": "%s: %s" % (w.name, w.description), "blocks": nblk, "zones": zones,
                    "form": "fused FP32 bulk, FP64 conserved accumulation", "solver": w.solver,
                    "limiter": w.limiter, "step_hbm_roofline_frac": (fused_zps * step_bytes / zones) / HBM_PEAK,
-                   "tuning": {"grid": best["grid"], "xcd_map": best["xcd_map"]}, "remesh": remesh},
+                   "tuning": {"grid": best["grid"], "xcd_map": best["xcd_map"],
+                              "curve": "hilbert" if best["curve"] else "morton"}, "remesh": remesh},
         "roofline": roof,
         "cpu_baseline": cpu,
         "plain_fp64": plain,

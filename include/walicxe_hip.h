@@ -195,8 +195,18 @@ int wx_admesh_plan(const int nroot[3], int maxlev, int nblk, const int *bids, co
    hold, such as a jump of two levels) with the mesh and state untouched. */
 int wx_admesh(void *ctx, double refine_thres, double coarse_thres, int cap, int *nblk_new, int *bids_new);
 
-/* Tile/launch choice of the fused form (autotuner): kind 0 = stage kernel
-   block size, value as documented in the implementation. */
+/* Hilbert key of every block (getHilbertKey, source/amr.f90:2031-2061, with
+   HKey3, source/hilbert.f90:38-127): the key of the block's lowest
+   finest-level cell on the reference's 3-D Hilbert curve; sorting by key
+   gives the reference's HilbertOrder (amr.f90:1930-1996), the order its
+   load balance deals blocks to ranks in.  Host only. */
+int wx_hilbert_keys(const int nroot[3], int maxlev, int nblk, const int *bids, int *keys);
+
+/* Launch and order choices of the fused form (autotuner): kind 0 = stage
+   kernel grid (0 one workgroup per block, else a persistent grid of that
+   many workgroups), kind 1 = block order split into one curve range per XCD
+   (1) or not (0), kind 2 = block order along the Morton (0) or Hilbert (1)
+   curve. */
 int wx_set_tuning(void *ctx, int kind, int value);
 /* Advances `reps` more iterations launching each stage kernel on its own
    between HIP events on the library's stream (fused form; no reference

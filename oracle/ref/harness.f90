@@ -37,7 +37,7 @@ program wx_oracle_harness
   use hydro_solver, only : hydroSolver
   use coolingModule, only : cooling, find_coef
   use load_balance, only : loadBalance
-  use amr, only : meshlevel, markByPhysical, admesh
+  use amr, only : meshlevel, markByPhysical, admesh, getHilbertKey
   use HLL, only : primfhll
   use HLLC, only : primfhllc
 #ifdef BFIELD
@@ -197,16 +197,21 @@ contains
   !  9 sound/cfast in p            out cs, cfx, cfy, cfz, cfastX
   ! 10 find_coef   in logT         out coef   (needs a cooling table)
   ! 11 calcTemp    in p            out T
+  ! 12 getHilbertKey in bID         out key (on the workload's mesh: needs ic.bin)
   subroutine unit_mode (opcode)
     integer, intent(in) :: opcode
     integer(4) :: n
-    integer :: r, istat, lim, dimn
+    integer :: r, istat, lim, dimn, hk
     real(8) :: a(neqtot), b(neqtot), c(neqtot), d(neqtot), f(neqtot), x, y
     real :: cs, cfx, cfy, cfz, cfX1
 
     if (opcode == 10) then
       if (cooling_type == COOL_NONE) stop 6
       call loadcooldata_wrap ()
+    end if
+    if (opcode == 12) then
+      call initmain ()
+      call basegrid ()
     end if
 
     open (unit=75, file='unit_in.bin', access='stream', form='unformatted', status='old')
@@ -271,6 +276,10 @@ contains
         read (75) a
         call calcTemp (a, x)
         write (76) x
+      case (12)
+        read (75) x
+        call getHilbertKey (nint(x), hk)
+        write (76) real(hk,8)
       case default
         stop 7
       end select

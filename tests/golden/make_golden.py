@@ -16,6 +16,7 @@ Fixtures (numpy .npz, no pickles):
                         of every block (f<s>_bids, f<s>_flags) and the block
                         list after admesh in slot order (b<s>), the state
                         summaries of the dumped steps in that order.
+  hkeys.npz             getHilbertKey of every block ID of two meshes.
   unit_<build>.npz      reference routine outputs on seeded inputs
                         (flow2prim, prim2flow, prim2fluxes, limiter, Riemann
                         solvers, wave speeds, find_coef, calcTemp).
@@ -109,6 +110,23 @@ def make_amr_traj(name):
             out["d%d_%s" % (d, k)] = v
     np.savez_compressed(os.path.join(HERE, "amr_%s.npz" % name), **out)
     print("amr  %-14s steps=%d dumps=%s (%.1fs)" % (name, n, dumps, time.time() - t0), flush=True)
+
+
+def make_hkeys():
+    """getHilbertKey of every block ID of two meshes: one root with five
+    levels (curve order 4) and 32x1x1 roots on one level (order 5)."""
+    out = {}
+    for name in HKEY_MESHES:
+        w = W.get(name)
+        bids = np.arange(1, w.mesh().maxid[-1] + 1)
+        k = refrun.run(w, 0, [], unit_op=12, unit_in=bids[:, None].astype(np.float64))
+        out["%s_bids" % name] = bids.astype(np.int32)
+        out["%s_keys" % name] = k.astype(np.int64)
+    np.savez_compressed(os.path.join(HERE, "hkeys.npz"), **out)
+    print("hkeys %s" % HKEY_MESHES, flush=True)
+
+
+HKEY_MESHES = ["amr_adapt", "briowu"]
 
 
 def rand_prims(rng, n, nv, mhd):
@@ -222,11 +240,14 @@ def main(argv):
             make_unit(n[5:], UNITS[n[5:]])
         elif n.startswith("amr:"):
             make_amr_traj(n[4:])
+        elif n == "hkeys":
+            make_hkeys()
         else:
             make_traj(n)
     if not argv:
         for n in AMR_TRAJ:
             make_amr_traj(n)
+        make_hkeys()
         for n, ops in UNITS.items():
             make_unit(n, ops)
 

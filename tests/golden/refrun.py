@@ -83,6 +83,9 @@ def run(w, nsteps, dumps, workdir=None, unit_op=None, unit_in=None, timeout=3600
         lt, lam = W.cooling_table()
         W.write_cooling_table(os.path.join(wd, "cooltable.dat"), lt, lam)
     if unit_op is not None:
+        if unit_op == 12:   # the mesh routines need the grid set up from ic.bin
+            slots, blocks, U = W.initial_state(w)
+            write_ic_with_passive(os.path.join(wd, "ic.bin"), w, blocks, U)
         with open(os.path.join(wd, "harness.nml"), "w") as f:
             f.write("&ctl\n mode='unit'\n op=%d\n/\n" % unit_op)
         with open(os.path.join(wd, "unit_in.bin"), "wb") as f:

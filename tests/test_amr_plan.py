@@ -9,6 +9,9 @@
 * Hand-built meshes whose outcome follows from the reference's rules by
   inspection, one per rule, along each axis, through both the oracle and the
   library.
+* Hilbert keys (wx_hilbert_keys) against getHilbertKey of the reference for
+  every block ID of two meshes (tests/golden/hkeys.npz), and the curve's own
+  property: consecutive keys are face neighbours.
 """
 import numpy as np
 import pytest
@@ -169,3 +172,32 @@ def test_plan_reports_the_size_when_the_capacity_is_short():
                                    flags.ctypes.data_as(hip._ip), None, 4, C.byref(n),
                                    out.ctypes.data_as(hip._ip))
     assert rc < 0 and n.value == 9
+
+
+# --------------------------------------------------------------------------
+# Hilbert keys (getHilbertKey / HKey3): the reference's keys, and the curve
+# property that fixes any table entry: consecutive keys are face neighbours
+# --------------------------------------------------------------------------
+@pytest.mark.parametrize("name", ["amr_adapt", "briowu"])
+def test_hilbert_keys_match_the_reference(name):
+    fx = load_golden("hkeys.npz")
+    m = W.get(name).mesh()
+    keys = hip.hilbert_keys(m, fx["%s_bids" % name])
+    assert np.array_equal(keys, fx["%s_keys" % name])
+
+
+@pytest.mark.parametrize("maxlev", [2, 3, 4, 5])
+def test_hilbert_curve_steps_between_face_neighbours(maxlev):
+    m = W.Mesh((1, 1, 1), maxlev, (1.0, 1.0, 1.0), 1.0)
+    finest = list(range(m.minid[-1], m.maxid[-1] + 1))
+    keys = hip.hilbert_keys(m, finest)
+    n = len(finest)
+    assert sorted(keys.tolist()) == list(range(n))
+    pos = np.array([m.coords(b) for b in finest])[np.argsort(keys)]
+    steps = np.abs(np.diff(pos, axis=0)).sum(axis=1)
+    assert np.all(steps == 1)
+    # a coarser block takes the key of its lowest finest-level cell
+    for b in range(m.minid[-2], m.maxid[-2] + 1):
+        x, y, z = m.coords(b)
+        low = m.bid(maxlev, 2 * x - 1, 2 * y - 1, 2 * z - 1)
+        assert hip.hilbert_keys(m, [b])[0] == hip.hilbert_keys(m, [low])[0]

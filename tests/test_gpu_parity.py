@@ -306,6 +306,34 @@ def test_fused_remesh_recaptures_graph():
     h.close()
 
 
+@pytest.mark.parametrize("name", ["amr_small", "ot64"])
+def test_fused_launch_and_order_choices_leave_results_unchanged(name):
+    """The autotuner's choices (persistent grid, XCD ranges, Morton or
+    Hilbert block order) change only which workgroup updates which block
+    when: every block's update reads the same data, and the dt reduction is
+    an exact minimum, so the state is bit-identical."""
+    from walicxe3d_amd import hip
+    w = W.get(name)
+    bids = np.asarray(w.blocks(), dtype=np.int32)
+    U0 = ic_in_order(w, bids)
+    say("case tuning neutrality", name, len(bids))
+    out = []
+    # persistent grids smaller than the block count: each workgroup updates
+    # several blocks in turn
+    for grid, xcd, curve in ((0, 0, 0), (32, 1, 1), (0, 1, 1), (48, 0, 0)):
+        g = gpu(w, hip.FUSED_FP32, bids)
+        g.set_tuning(0, grid)
+        g.set_tuning(1, xcd)
+        g.set_tuning(2, curve)
+        g.upload(U0)
+        g.run(4)
+        out.append((g.dt.value, g.download(prim=False)))
+        g.close()
+    for dt, u in out[1:]:
+        assert dt == out[0][0]
+        assert np.array_equal(u, out[0][1])
+
+
 @pytest.mark.parametrize("form", ["fused_fp32", "plain"])
 def test_device_remesh_matches_host_regrid(form):
     """wx_remesh carries the state to a new block list on the device (copy,

@@ -15,6 +15,7 @@
 
 #include "wx_device.h"
 #include "wx_fused.h"
+#include "wx_hilbert.h"
 #include "wx_kernels.h"
 
 using namespace wx;
@@ -714,6 +715,7 @@ int wx_set_mesh(void *ctx, const int nroot[3], int maxlev, const double *dxyz, i
   fill_devcfg(c, dxyz);
   if (is_fused(c)) {
     c->fused.fp64 = c->cfg.form == WX_FORM_FUSED_FP64;
+    for (int d = 0; d < 3; d++) c->fused.nroot[d] = c->nroot[d];
     int frc = fused_alloc(c->fused, c->dc, c->info, c->st);
     if (frc != 0) return fail(WX_ENOMEM, "fused-form allocation failed");
   }
@@ -1077,6 +1079,22 @@ int wx_admesh_plan(const int nroot[3], int maxlev, int nblk, const int *bids, co
   if ((int)out.size() > cap || !bids_new)
     return fail(WX_ENOMEM, "new block list (" + std::to_string(out.size()) + " blocks) exceeds the capacity");
   std::copy(out.begin(), out.end(), bids_new);
+  return WX_OK;
+}
+
+int wx_hilbert_keys(const int nroot[3], int maxlev, int nblk, const int *bids, int *keys) {
+  if (!nroot || !bids || !keys || nblk < 0 || maxlev < 1 || maxlev > MAXLEV) return fail(WX_EINVAL, "bad argument");
+  Numbering N;
+  N.init(nroot, maxlev);
+  const int order = hilbert_order(nroot, maxlev);
+  if (3 * order > 31) return fail(WX_EINVAL, "mesh too large for 31-bit Hilbert keys");
+  for (int n = 0; n < nblk; n++) {
+    const int lev = N.level(bids[n]);
+    if (lev < 0) return fail(WX_ETOPO, "block ID outside the mesh: " + std::to_string(bids[n]));
+    int c[3];
+    N.coords(bids[n], lev, c);
+    keys[n] = (int)block_hilbert_key(c, lev, maxlev, order);
+  }
   return WX_OK;
 }
 

@@ -33,6 +33,8 @@ struct FusedState {
   int nblk = 0, nv = 0;
   int grid = 0;              // stage-kernel grid size (0 = one workgroup per block)
   int xcd_map = 1;           // XCD-aware block order
+  int curve = 0;             // block order along the Morton (0) or Hilbert (1) curve
+  int nroot[3] = {1, 1, 1};  // root blocks per axis (Hilbert curve order)
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};

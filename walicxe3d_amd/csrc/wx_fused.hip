@@ -25,6 +25,7 @@
 #include <string>
 
 #include "wx_fused.h"
+#include "wx_hilbert.h"
 #include "wx_fused_kern.h"
 
 namespace wx {
@@ -110,13 +111,20 @@ static unsigned long long morton(const BlockInfo &B, int maxlev) {
   return key;
 }
 
-// Block processing order: Morton order of the blocks' positions; with the
-// XCD map, workgroup w runs on XCD w % 8, so each XCD gets a contiguous
-// Morton range (neighbour halos then hit that XCD's L2).
+// Block processing order: Morton (or Hilbert) order of the blocks'
+// positions; with the XCD map, workgroup w runs on XCD w % 8, so each XCD
+// gets a contiguous range of the curve (neighbour halos then hit that XCD's
+// L2).
 static int build_order(FusedState &f) {
   const int nb = f.nblk;
   std::vector<std::pair<unsigned long long, int>> keys(nb);
-  for (int b = 0; b < nb; b++) keys[b] = {morton(f.host_info[b], f.maxlev), b};
+  if (f.curve == 1) {
+    const int order = hilbert_order(f.nroot, f.maxlev);
+    for (int b = 0; b < nb; b++)
+      keys[b] = {block_hilbert_key(f.host_info[b].coords, f.host_info[b].level, f.maxlev, order), b};
+  } else {
+    for (int b = 0; b < nb; b++) keys[b] = {morton(f.host_info[b], f.maxlev), b};
+  }
   std::sort(keys.begin(), keys.end());
   const int nx = 8;
   std::vector<int> ord(nb, -1);
@@ -320,9 +328,11 @@ int fused_time_stages(FusedState &f, const DevCfg &c, hipStream_t st, int reps) 
 
 // kind 0: stage-kernel grid (0 = one workgroup per block, else a persistent
 //         grid of that many workgroups striding over the block list)
-// kind 1: XCD-aware block order (1) or plain Morton order (0)
+// kind 1: XCD-aware block order (1) or one range for all XCDs (0)
+// kind 2: block order along the Morton (0) or the Hilbert (1) curve
 int fused_set_tuning(FusedState &f, int kind, int value) {
-  if (kind != 0 && kind != 1) return 1;
+  if (kind < 0 || kind > 2) return 1;
+  if (kind == 2 && (value < 0 || value > 1)) return 1;
   if (f.exec) {
     hipGraphExecDestroy(f.exec);
     hipGraphDestroy(f.graph);
@@ -334,7 +344,10 @@ int fused_set_tuning(FusedState &f, int kind, int value) {
     f.grid = value;
     return 0;
   }
-  f.xcd_map = value != 0;
+  if (kind == 1)
+    f.xcd_map = value != 0;
+  else
+    f.curve = value;
   return f.nblk > 0 ? build_order(f) : 0;
 }
 

@@ -79,6 +79,14 @@ module walicxe_hip
       integer(c_int), intent(out) :: flags(*)
     end function
 
+    integer(c_int) function wx_hilbert_keys(nroot, maxlev, nblk, bids, keys) bind(C, name="wx_hilbert_keys")
+      import :: c_int
+      integer(c_int), intent(in) :: nroot(3)
+      integer(c_int), value :: maxlev, nblk
+      integer(c_int), intent(in) :: bids(*)
+      integer(c_int), intent(out) :: keys(*)
+    end function
+
     integer(c_int) function wx_record_ghosts(ctx) bind(C, name="wx_record_ghosts")
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx

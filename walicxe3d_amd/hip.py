@@ -70,6 +70,7 @@ SIGNATURES = {
     "wx_admesh_plan": (C.c_int, [_ip, C.c_int, C.c_int, _ip, _ip, _ip, C.c_int, _ip, _ip]),
     "wx_admesh": (C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_int, _ip, _ip]),
     "wx_record_ghosts": (C.c_int, [C.c_void_p]),
+    "wx_hilbert_keys": (C.c_int, [_ip, C.c_int, C.c_int, _ip, _ip]),
 }
 
 FLAG_NONE, FLAG_REFINE, FLAG_COARSE = 0, 1, 2
@@ -125,6 +126,16 @@ def admesh_plan(mesh, bids, flags):
                                  fo.ctypes.data_as(_ip), cap, C.byref(n), out.ctypes.data_as(_ip)),
            "wx_admesh_plan")
     return out[:n.value].copy(), fo
+
+
+def hilbert_keys(mesh, bids):
+    """getHilbertKey of every block (wx_hilbert_keys)."""
+    b = np.ascontiguousarray(bids, dtype=np.int32)
+    k = np.empty_like(b)
+    nroot = (C.c_int * 3)(*mesh.nroot)
+    _check(load().wx_hilbert_keys(nroot, mesh.maxlev, len(b), b.ctypes.data_as(_ip), k.ctypes.data_as(_ip)),
+           "wx_hilbert_keys")
+    return k
 
 
 def make_config(w, form=PLAIN_FP64, table=None, winds=None):
