@@ -31,6 +31,7 @@ extern "C" {
 #define WX_ENOMEM (-3)
 #define WX_ESTATE (-4)
 #define WX_ETOPO (-5)
+#define WX_EIO (-6)
 
 /* numerical forms */
 #define WX_FORM_PLAIN_FP64 0   /* one kernel per reference routine, FP64, reference order */
@@ -201,6 +202,24 @@ int wx_admesh(void *ctx, double refine_thres, double coarse_thres, int cap, int 
    gives the reference's HilbertOrder (amr.f90:1930-1996), the order its
    load balance deals blocks to ranks in.  Host only. */
 int wx_hilbert_keys(const int nroot[3], int maxlev, int nblk, const int *bids, int *keys);
+
+/* Checkpoint files of the reference, byte for byte.  writeBin
+   (source/output.f90:156-214): the device state to a stream file (int32
+   block count; per block int32 bID and U(1:neqtot,1:16,1:16,1:16), variable
+   fastest), neqtot = nv + npassive with the passive columns, which the
+   library does not carry, written as zero. */
+int wx_write_bin(void *ctx, const char *path, int npassive);
+/* warmstart's read of that file (source/init.f90:582-610) into host arrays:
+   bids[cap] and u[cap][nv][16][16][16] (WX_LAYOUT_BLOCKS), passive columns
+   skipped; then wx_set_mesh + wx_upload_state.  *nblk is set even when the
+   file holds more than cap blocks (WX_ENOMEM).  Host only. */
+int wx_read_bin(const char *path, int nv, int npassive, int cap, int *nblk, int *bids, double *u);
+/* writeState (source/output.f90:219-268): time*t_sc, it, noutput as
+   '(es22.15,i8,i5)', then the data directory.  Host only. */
+int wx_write_state(const char *path, double time, double t_sc, int it, int noutput, const char *datadir);
+/* warmstart's read of it (source/init.f90:545-560): time = t/t_sc,
+   nextout = noutput + 1, and the data directory (len bytes). */
+int wx_read_state(const char *path, double t_sc, double *time, int *it, int *nextout, char *datadir, int len);
 
 /* Launch and order choices of the fused form (autotuner): kind 0 = stage
    kernel grid (0 one workgroup per block, else a persistent grid of that

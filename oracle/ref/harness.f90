@@ -20,6 +20,10 @@
 !                     flag pairs) and afterwards the active bIDs in slot order
 !                     (blocks_NNNNN.bin: nact, then bIDs), the order in which
 !                     the state dumps of that step are written.
+!   bindump = .true. : at every dumped step also the reference's own output
+!                     routines writeBin and writeState (source/output.f90:
+!                     156-268) with the step as output number
+!                     (Blocks000.NNNN.bin, State.NNNN.dat)
 !   nsteps, ndump, dumps(:)  - steps after which U is written (0 = initial)
 !   tstop  - if > 0, stop (and write the state) after the first step that
 !            reaches time >= tstop
@@ -38,6 +42,7 @@ program wx_oracle_harness
   use coolingModule, only : cooling, find_coef
   use load_balance, only : loadBalance
   use amr, only : meshlevel, markByPhysical, admesh, getHilbertKey
+  use output, only : writeBin, writeState
   use HLL, only : primfhll
   use HLLC, only : primfhllc
 #ifdef BFIELD
@@ -51,8 +56,8 @@ program wx_oracle_harness
   integer :: dumps(64) = -1
   integer :: op = 0
   real(8) :: tstop = -1.0
-  logical :: amr = .false.
-  namelist /ctl/ mode, nsteps, ndump, dumps, op, tstop, amr
+  logical :: amr = .false., bindump = .false.
+  namelist /ctl/ mode, nsteps, ndump, dumps, op, tstop, amr, bindump
 
   integer :: s
 
@@ -78,6 +83,7 @@ program wx_oracle_harness
     open (unit=71, file='steps.txt', status='replace', action='write')
   end if
   if (nProcs == 1 .and. any(dumps(1:max(ndump,1)) == 0)) call write_state (0)
+  if (bindump .and. any(dumps(1:max(ndump,1)) == 0)) call write_output_files (0)
 
   do s=1,nsteps
     it = it + 1
@@ -96,6 +102,7 @@ program wx_oracle_harness
       dumpout = .false.
     end if
     if (nProcs == 1 .and. any(dumps(1:max(ndump,1)) == s)) call write_state (s)
+    if (bindump .and. any(dumps(1:max(ndump,1)) == s)) call write_output_files (s)
     if (tstop > 0.0 .and. time >= tstop) then
       if (nProcs == 1 .and. .not. any(dumps(1:max(ndump,1)) == s)) call write_state (s)
       exit
@@ -125,6 +132,14 @@ contains
     write (72) gamma, cv, CFL, visc_eta, d_sc, e_sc, t_sc, p_sc, l_sc, v_sc
     close (72)
   end subroutine write_topology
+
+  !-----------------------------------------------------------------------------
+  ! the reference's checkpoint files for output number `step`
+  subroutine write_output_files (step)
+    integer, intent(in) :: step
+    call writeBin (step)
+    if (rank == master) call writeState (step)
+  end subroutine write_output_files
 
   !-----------------------------------------------------------------------------
   ! markByPhysical (source/amr.f90:142-230) of every active block, before admesh

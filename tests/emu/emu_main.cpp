@@ -89,6 +89,9 @@ static int run_case(const char *path, int nsteps) {
   std::vector<double> Uo(U.size());
   wx_download_state(ctx, Uo.data(), nullptr, WX_LAYOUT_BLOCKS, 0, nullptr);
   { std::string fn = std::string(path) + ".out"; FILE *fo = fopen(fn.c_str(), "wb"); fwrite(Uo.data(), 8, Uo.size(), fo); fclose(fo); }
+  // the reference's checkpoint file of the final state (one zero passive
+  // column, as the oracle builds of the reference carry one passive scalar)
+  { std::string fn = std::string(path) + ".wxbin"; if (wx_write_bin(ctx, fn.c_str(), 1) < 0) { printf("write_bin %s\n", wx_last_error()); return 1; } }
   double s = 0; for (double x : Uo) s += x;
   printf("it %d dt %.17g time %.17g sum %.17g\n", it, dt, time, s);
   wx_finalize(ctx);

@@ -17,6 +17,8 @@ Fixtures (numpy .npz, no pickles):
                         list after admesh in slot order (b<s>), the state
                         summaries of the dumped steps in that order.
   hkeys.npz             getHilbertKey of every block ID of two meshes.
+  io.npz                the reference's checkpoint files (writeBin: SHA-256
+                        and size; writeState: text) at a few steps.
   unit_<build>.npz      reference routine outputs on seeded inputs
                         (flow2prim, prim2flow, prim2fluxes, limiter, Riemann
                         solvers, wave speeds, find_coef, calcTemp).
@@ -127,6 +129,29 @@ def make_hkeys():
 
 
 HKEY_MESHES = ["amr_adapt", "briowu"]
+
+
+def make_io():
+    """The reference's own checkpoint files (writeBin, writeState) at a few
+    steps: SHA-256 and size of each blocks file (the files themselves are
+    several MB), the state file's text, and dt/time of the run."""
+    import hashlib
+    out = {}
+    for name, (n, dumps) in IO_RUNS.items():
+        w = W.get(name)
+        r = refrun.run(w, n, dumps, bindump=True)
+        out["%s_steps" % name] = r["steps"]
+        out["%s_dumps" % name] = np.asarray(dumps, dtype=np.int32)
+        for d in dumps:
+            b, t = r["files"][d]
+            out["%s_d%d_sha256" % (name, d)] = np.asarray(hashlib.sha256(b).hexdigest())
+            out["%s_d%d_size" % (name, d)] = np.asarray(len(b), dtype=np.int64)
+            out["%s_d%d_state" % (name, d)] = np.asarray(t)
+    np.savez_compressed(os.path.join(HERE, "io.npz"), **out)
+    print("io %s" % list(IO_RUNS), flush=True)
+
+
+IO_RUNS = {"ot64": (3, [0, 3]), "amr_small": (2, [0, 2])}
 
 
 def rand_prims(rng, n, nv, mhd):
@@ -242,12 +267,15 @@ def main(argv):
             make_amr_traj(n[4:])
         elif n == "hkeys":
             make_hkeys()
+        elif n == "io":
+            make_io()
         else:
             make_traj(n)
     if not argv:
         for n in AMR_TRAJ:
             make_amr_traj(n)
         make_hkeys()
+        make_io()
         for n, ops in UNITS.items():
             make_unit(n, ops)
 

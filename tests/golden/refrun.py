@@ -62,14 +62,16 @@ def read_ints(path):
 
 
 def run(w, nsteps, dumps, workdir=None, unit_op=None, unit_in=None, timeout=3600, tstop=-1.0, variant="",
-        nprocs=1, amr=False):
+        nprocs=1, amr=False, bindump=False):
     """Run the harness; returns dict with topo, steps (it, dt, time, tout),
     and states {step: (U, PRIM)} with the passive scalar stripped.  With
     nprocs > 1 the multi-rank build runs under mpiexec and only the step log
     comes back (no topology, no states).  With amr the adaptive-mesh build
     runs admesh after every step; then also flags {step: (bids, flags)}
     (markByPhysical before admesh) and blocks {step: bids after admesh, in
-    the order of that step's state dump}."""
+    the order of that step's state dump}.  With bindump the reference's
+    own checkpoint files of each dumped step come back as
+    files {step: (Blocks000.NNNN.bin bytes, State.NNNN.dat text)}."""
     if nprocs > 1:
         variant = variant + "_np%d" % nprocs
     if amr:
@@ -98,7 +100,8 @@ def run(w, nsteps, dumps, workdir=None, unit_op=None, unit_in=None, timeout=3600
     dl = ",".join(str(d) for d in dumps)
     with open(os.path.join(wd, "harness.nml"), "w") as f:
         f.write("&ctl\n mode='run'\n nsteps=%d\n ndump=%d\n dumps=%s\n tstop=%r\n%s/\n"
-                % (nsteps, len(dumps), dl or "-1", float(tstop), " amr=.true.\n" if amr else ""))
+                % (nsteps, len(dumps), dl or "-1", float(tstop),
+                   (" amr=.true.\n" if amr else "") + (" bindump=.true.\n" if bindump else "")))
     cmd = [exe] if nprocs == 1 else [MPIEXEC, "-n", str(nprocs), exe]
     r = subprocess.run(cmd, cwd=wd, timeout=timeout, capture_output=True, text=True)
     if r.returncode != 0:
@@ -127,6 +130,14 @@ def run(w, nsteps, dumps, workdir=None, unit_op=None, unit_in=None, timeout=3600
         p = read_state(os.path.join(wd, "prim_%05d.bin" % d), nb, w.nv + 1)[:, : w.nv]
         states[d] = (u.copy(), p.copy())
     out = dict(topo=topo, steps=steps, states=states, ic_blocks=blocks, ic_U=U, dumps=list(dumps))
+    if bindump:
+        files = {}
+        for d in dumps:
+            with open(os.path.join(wd, "Blocks000.%04d.bin" % d), "rb") as f:
+                b = f.read()
+            with open(os.path.join(wd, "State.%04d.dat" % d)) as f:
+                files[d] = (b, f.read())
+        out["files"] = files
     if amr:
         out.update(flags=flags, blocks=blocks_after)
     if own:

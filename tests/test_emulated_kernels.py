@@ -36,7 +36,7 @@ def _build():
              "-Wno-unknown-pragmas", "-I" + EMU, "-I" + os.path.join(ROOT, "walicxe3d_amd", "csrc"),
              "-I" + os.path.join(ROOT, "include")]
     objs = []
-    for src in ("wx_api", "wx_fused", "wx_fused_mhd", "wx_plain"):
+    for src in ("wx_api", "wx_fused", "wx_fused_mhd", "wx_plain", "wx_io"):
         o = os.path.join(BUILD, src + ".o")
         cmd = [CXX, "-x", "c++"] + flags + ["-c", os.path.join(ROOT, "walicxe3d_amd", "csrc", src + ".hip"),
                                             "-o", o]
@@ -117,6 +117,16 @@ def test_emulation_reproduces_fused_fp64(emu, tmp_path, oracle):
     uo, _ = o.get_state()
     ue = np.fromfile(p + ".out").reshape(uo.shape)
     assert np.array_equal(ue, uo)
+    # wx_write_bin of that state is the reference's own writeBin file of step 3
+    # byte for byte, and wx_read_bin reads it back
+    import hashlib
+    from conftest import load_golden
+    fx = load_golden("io.npz")
+    data = open(p + ".wxbin", "rb").read()
+    assert len(data) == int(fx["ot64_d3_size"])
+    assert hashlib.sha256(data).hexdigest() == str(fx["ot64_d3_sha256"])
+    bids, ur = hip.read_bin(p + ".wxbin", 8, npassive=1)
+    assert np.array_equal(bids, np.asarray(blocks)) and np.array_equal(ur, uo)
 
 
 def test_emulated_admesh_reproduces_the_oracle(emu, tmp_path, oracle):

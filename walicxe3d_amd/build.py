@@ -27,6 +27,7 @@ COMMON = ["-O3", "--offload-arch=" + ARCH, "-fPIC", "-std=c++17", "-I" + CSRC, "
 UNITS = {
     "wx_plain.hip": ["-ffp-contract=off"],
     "wx_api.hip": ["-ffp-contract=off"],
+    "wx_io.hip": ["-ffp-contract=off"],
     # no contraction either: a face flux computed by the two blocks that share
     # it must round identically for the update to stay conservative
     # FP32 division and square root with the hardware's approximate lowering

@@ -16,6 +16,7 @@
 #include "wx_device.h"
 #include "wx_fused.h"
 #include "wx_hilbert.h"
+#include "wx_io.h"
 #include "wx_kernels.h"
 
 using namespace wx;
@@ -1095,6 +1096,46 @@ int wx_hilbert_keys(const int nroot[3], int maxlev, int nblk, const int *bids, i
     N.coords(bids[n], lev, c);
     keys[n] = (int)block_hilbert_key(c, lev, maxlev, order);
   }
+  return WX_OK;
+}
+
+int wx_write_bin(void *ctx, const char *path, int npassive) {
+  Ctx *c = as_ctx(ctx);
+  int rc = need_state(c);
+  if (rc != WX_OK) return rc;
+  if (!path || npassive < 0) return fail(WX_EINVAL, "bad argument");
+  std::vector<double> u((size_t)c->nblk * c->cfg.nv * NINT);
+  rc = wx_download_state(ctx, u.data(), nullptr, WX_LAYOUT_BLOCKS, 0, nullptr);
+  if (rc < 0) return rc;
+  std::string err;
+  if (bin_write(path, c->nblk, c->cfg.nv, npassive, c->bids.data(), u.data(), err)) return fail(WX_EIO, err);
+  return WX_OK;
+}
+
+int wx_read_bin(const char *path, int nv, int npassive, int cap, int *nblk, int *bids, double *u) {
+  if (!path || !nblk || (nv != 5 && nv != 8) || npassive < 0) return fail(WX_EINVAL, "bad argument");
+  std::string err;
+  const int r = bin_read(path, nv, npassive, cap, nblk, bids, u, err);
+  if (r == 2) return fail(WX_ENOMEM, err);
+  if (r) return fail(WX_EIO, err);
+  return WX_OK;
+}
+
+int wx_write_state(const char *path, double time, double t_sc, int it, int noutput, const char *datadir) {
+  if (!path) return fail(WX_EINVAL, "null path");
+  std::string err;
+  if (state_write(path, time * t_sc, it, noutput, datadir, err)) return fail(WX_EIO, err);
+  return WX_OK;
+}
+
+int wx_read_state(const char *path, double t_sc, double *time, int *it, int *nextout, char *datadir, int len) {
+  if (!path || !time || !it || !nextout) return fail(WX_EINVAL, "null argument");
+  std::string err;
+  double tp = 0.0;
+  int nout = 0;
+  if (state_read(path, &tp, it, &nout, datadir, len, err)) return fail(WX_EIO, err);
+  *time = tp / t_sc;
+  *nextout = nout + 1;
   return WX_OK;
 }
 

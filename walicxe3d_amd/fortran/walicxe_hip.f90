@@ -9,6 +9,7 @@ module walicxe_hip
   implicit none
 
   integer(c_int), parameter :: WX_OK = 0
+  integer(c_int), parameter :: WX_EIO = -6
   integer(c_int), parameter :: WX_FORM_PLAIN_FP64 = 0
   integer(c_int), parameter :: WX_FORM_FUSED_FP32 = 1
   integer(c_int), parameter :: WX_FORM_FUSED_FP64 = 2
@@ -85,6 +86,39 @@ module walicxe_hip
       integer(c_int), value :: maxlev, nblk
       integer(c_int), intent(in) :: bids(*)
       integer(c_int), intent(out) :: keys(*)
+    end function
+
+    integer(c_int) function wx_write_bin(ctx, path, npassive) bind(C, name="wx_write_bin")
+      import :: c_int, c_ptr, c_char
+      type(c_ptr), value :: ctx
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int), value :: npassive
+    end function
+
+    integer(c_int) function wx_read_bin(path, nv, npassive, cap, nblk, bids, u) bind(C, name="wx_read_bin")
+      import :: c_int, c_char, c_double
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int), value :: nv, npassive, cap
+      integer(c_int), intent(out) :: nblk
+      integer(c_int), intent(out) :: bids(*)
+      real(c_double), intent(out) :: u(*)
+    end function
+
+    integer(c_int) function wx_write_state(path, time, t_sc, it, noutput, datadir) bind(C, name="wx_write_state")
+      import :: c_int, c_char, c_double
+      character(kind=c_char), intent(in) :: path(*), datadir(*)
+      real(c_double), value :: time, t_sc
+      integer(c_int), value :: it, noutput
+    end function
+
+    integer(c_int) function wx_read_state(path, t_sc, time, it, nextout, datadir, len) bind(C, name="wx_read_state")
+      import :: c_int, c_char, c_double
+      character(kind=c_char), intent(in) :: path(*)
+      real(c_double), value :: t_sc
+      real(c_double), intent(out) :: time
+      integer(c_int), intent(out) :: it, nextout
+      character(kind=c_char), intent(out) :: datadir(*)
+      integer(c_int), value :: len
     end function
 
     integer(c_int) function wx_record_ghosts(ctx) bind(C, name="wx_record_ghosts")

@@ -71,6 +71,10 @@ SIGNATURES = {
     "wx_admesh": (C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_int, _ip, _ip]),
     "wx_record_ghosts": (C.c_int, [C.c_void_p]),
     "wx_hilbert_keys": (C.c_int, [_ip, C.c_int, C.c_int, _ip, _ip]),
+    "wx_write_bin": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int]),
+    "wx_read_bin": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_int, _ip, _ip, _dp]),
+    "wx_write_state": (C.c_int, [C.c_char_p, C.c_double, C.c_double, C.c_int, C.c_int, C.c_char_p]),
+    "wx_read_state": (C.c_int, [C.c_char_p, C.c_double, _dp, _ip, _ip, C.c_char_p, C.c_int]),
 }
 
 FLAG_NONE, FLAG_REFINE, FLAG_COARSE = 0, 1, 2
@@ -136,6 +140,33 @@ def hilbert_keys(mesh, bids):
     _check(load().wx_hilbert_keys(nroot, mesh.maxlev, len(b), b.ctypes.data_as(_ip), k.ctypes.data_as(_ip)),
            "wx_hilbert_keys")
     return k
+
+
+def read_bin(path, nv, npassive=0, cap=None):
+    """The reference's blocks file (wx_read_bin): (bids, U [nblk][nv][16]^3)."""
+    n = C.c_int(0)
+    if cap is None:   # the header gives the count
+        with open(path, "rb") as f:
+            cap = int(np.frombuffer(f.read(4), dtype="<i4")[0])
+    bids = np.empty(max(cap, 1), dtype=np.int32)
+    u = np.empty((max(cap, 1), nv, 16, 16, 16))
+    _check(load().wx_read_bin(os.fsencode(path), nv, npassive, cap, C.byref(n), bids.ctypes.data_as(_ip), _p(u)),
+           "wx_read_bin")
+    return bids[:n.value].copy(), u[:n.value].copy()
+
+
+def write_state(path, time, t_sc, it, noutput, datadir="./"):
+    """writeState's file (wx_write_state); time in code units."""
+    _check(load().wx_write_state(os.fsencode(path), time, t_sc, it, noutput, datadir.encode()), "wx_write_state")
+
+
+def read_state(path, t_sc):
+    """warmstart's read (wx_read_state): (time in code units, it, nextout, datadir)."""
+    t, it, nx = C.c_double(0.0), C.c_int(0), C.c_int(0)
+    buf = C.create_string_buffer(256)
+    _check(load().wx_read_state(os.fsencode(path), t_sc, C.byref(t), C.byref(it), C.byref(nx), buf, 256),
+           "wx_read_state")
+    return t.value, it.value, nx.value, buf.value.decode()
 
 
 def make_config(w, form=PLAIN_FP64, table=None, winds=None):
@@ -210,6 +241,10 @@ class Walicxe3D:
         self.bids = b
         self.nblk = len(b)
         return rc
+
+    def writeBin(self, path, npassive=0):
+        """The reference's blocks file of the current state (wx_write_bin)."""
+        return _check(load().wx_write_bin(self._h, os.fsencode(path), npassive), "wx_write_bin")
 
     def record_ghosts(self):
         """Fused form: keep the ghost layer of the current state for the next
