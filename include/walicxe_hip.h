@@ -221,6 +221,16 @@ int wx_write_state(const char *path, double time, double t_sc, int it, int noutp
    nextout = noutput + 1, and the data directory (len bytes). */
 int wx_read_state(const char *path, double t_sc, double *time, int *it, int *nextout, char *datadir, int len);
 
+/* The VTK part of writeOutput (source/output.f90:38-151) on one rank:
+   updatePrims, then write3DVTKBlocks (:273-750: datadir/Blocks000.NNNN.vtk,
+   every cell a hexahedron, rho, vel, P, B, T and npassive passive scalars
+   (zero, not carried), FP32 big-endian; master.visit replaced at output 0
+   and appended to after) and write3DVTKGrid (:755-895: datadir/Grid.NNNN.vtk,
+   one hexahedron per block with rank, bID and Hilbert key).  units is the
+   reference's units_type: 1 PHYS_UNITS (scaled by l_sc, d_sc, v_sc, p_sc,
+   b_sc), 2 CODE_UNITS. */
+int wx_write_vtk(void *ctx, const char *datadir, int noutput, int npassive, int units);
+
 /* Launch and order choices of the fused form (autotuner): kind 0 = stage
    kernel grid (0 one workgroup per block, else a persistent grid of that
    many workgroups), kind 1 = block order split into one curve range per XCD

@@ -100,6 +100,7 @@ def parameters_text(w, nprocs=1, amr=False):
         "GAMMA": w.gamma_expr, "MU0": flit(w.mu0), "MUI": flit(w.mui),
         "ION_THRES": flit(w.ion_thres), "EOS": EOS[w.eos],
         "L_SC": w.l_sc_expr, "D_SC": w.d_sc_expr, "V_SC": w.v_sc_expr,
+        "UNITS": "PHYS_UNITS" if w.units == "phys" else "CODE_UNITS",
     }
     with open(os.path.join(HERE, "ref", "parameters.f90.in")) as f:
         text = f.read()

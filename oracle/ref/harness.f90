@@ -24,6 +24,11 @@
 !                     routines writeBin and writeState (source/output.f90:
 !                     156-268) with the step as output number
 !                     (Blocks000.NNNN.bin, State.NNNN.dat)
+!   vtkdump = .true. : at every dumped step also the reference's VTK output
+!                     as writeOutput does it (updatePrims, write3DVTKBlocks,
+!                     write3DVTKGrid: Blocks000.NNNN.vtk, Grid.NNNN.vtk,
+!                     master.visit); updatePrims replaces the interior PRIM,
+!                     so later steps are those of a run with VTK output
 !   nsteps, ndump, dumps(:)  - steps after which U is written (0 = initial)
 !   tstop  - if > 0, stop (and write the state) after the first step that
 !            reaches time >= tstop
@@ -42,7 +47,7 @@ program wx_oracle_harness
   use coolingModule, only : cooling, find_coef
   use load_balance, only : loadBalance
   use amr, only : meshlevel, markByPhysical, admesh, getHilbertKey
-  use output, only : writeBin, writeState
+  use output, only : writeBin, writeState, write3DVTKBlocks, write3DVTKGrid
   use HLL, only : primfhll
   use HLLC, only : primfhllc
 #ifdef BFIELD
@@ -56,8 +61,8 @@ program wx_oracle_harness
   integer :: dumps(64) = -1
   integer :: op = 0
   real(8) :: tstop = -1.0
-  logical :: amr = .false., bindump = .false.
-  namelist /ctl/ mode, nsteps, ndump, dumps, op, tstop, amr, bindump
+  logical :: amr = .false., bindump = .false., vtkdump = .false.
+  namelist /ctl/ mode, nsteps, ndump, dumps, op, tstop, amr, bindump, vtkdump
 
   integer :: s
 
@@ -83,7 +88,7 @@ program wx_oracle_harness
     open (unit=71, file='steps.txt', status='replace', action='write')
   end if
   if (nProcs == 1 .and. any(dumps(1:max(ndump,1)) == 0)) call write_state (0)
-  if (bindump .and. any(dumps(1:max(ndump,1)) == 0)) call write_output_files (0)
+  if ((bindump .or. vtkdump) .and. any(dumps(1:max(ndump,1)) == 0)) call write_output_files (0)
 
   do s=1,nsteps
     it = it + 1
@@ -102,7 +107,7 @@ program wx_oracle_harness
       dumpout = .false.
     end if
     if (nProcs == 1 .and. any(dumps(1:max(ndump,1)) == s)) call write_state (s)
-    if (bindump .and. any(dumps(1:max(ndump,1)) == s)) call write_output_files (s)
+    if ((bindump .or. vtkdump) .and. any(dumps(1:max(ndump,1)) == s)) call write_output_files (s)
     if (tstop > 0.0 .and. time >= tstop) then
       if (nProcs == 1 .and. .not. any(dumps(1:max(ndump,1)) == s)) call write_state (s)
       exit
@@ -137,8 +142,15 @@ contains
   ! the reference's checkpoint files for output number `step`
   subroutine write_output_files (step)
     integer, intent(in) :: step
-    call writeBin (step)
-    if (rank == master) call writeState (step)
+    if (bindump) then
+      call writeBin (step)
+      if (rank == master) call writeState (step)
+    end if
+    if (vtkdump) then
+      call updatePrims ()
+      call write3DVTKBlocks (step)
+      if (rank == master) call write3DVTKGrid (step)
+    end if
   end subroutine write_output_files
 
   !-----------------------------------------------------------------------------

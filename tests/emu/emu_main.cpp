@@ -3,7 +3,9 @@
 // pair is an input file written by tests/test_emulated_kernels.py and a step
 // count; the final state goes to <input>.out.  A remesh count of -1 in the
 // input asks for wx_admesh after every step instead (final block list to
-// <input>.blocks).
+// <input>.blocks).  With EMU_VTK_DIR set, writeOutput's VTK files of the
+// initial and the final state (output numbers 0 and nsteps, one passive
+// column, units_type EMU_VTK_UNITS, default CODE_UNITS) go to that folder.
 #include <cstdio>
 #include <cstdlib>
 #include <vector>
@@ -49,6 +51,9 @@ static int run_case(const char *path, int nsteps) {
   int rc = wx_init(&cfg, &ctx); if (rc < 0) { printf("init %d %s\n", rc, wx_last_error()); return 1; }
   rc = wx_set_mesh(ctx, nroot, maxlev, dx.data(), nblk, bids.data()); if (rc < 0) { printf("mesh %d %s\n", rc, wx_last_error()); return 1; }
   rc = wx_upload_state(ctx, U.data(), WX_LAYOUT_BLOCKS, 0, nullptr); if (rc < 0) { printf("upload %d %s\n", rc, wx_last_error()); return 1; }
+  const char *vtkdir = getenv("EMU_VTK_DIR");
+  const int vtkunits = getenv("EMU_VTK_UNITS") ? atoi(getenv("EMU_VTK_UNITS")) : 2;
+  if (vtkdir && wx_write_vtk(ctx, vtkdir, 0, 1, vtkunits) < 0) { printf("vtk %s\n", wx_last_error()); return 1; }
   int it = 0, nextout = 1; double dt = 0, time = 0;
   if (nb2 == -1) {
     std::vector<int> nb(8 * 4096);
@@ -89,6 +94,7 @@ static int run_case(const char *path, int nsteps) {
   std::vector<double> Uo(U.size());
   wx_download_state(ctx, Uo.data(), nullptr, WX_LAYOUT_BLOCKS, 0, nullptr);
   { std::string fn = std::string(path) + ".out"; FILE *fo = fopen(fn.c_str(), "wb"); fwrite(Uo.data(), 8, Uo.size(), fo); fclose(fo); }
+  if (vtkdir && wx_write_vtk(ctx, vtkdir, nsteps, 1, vtkunits) < 0) { printf("vtk %s\n", wx_last_error()); return 1; }
   // the reference's checkpoint file of the final state (one zero passive
   // column, as the oracle builds of the reference carry one passive scalar)
   { std::string fn = std::string(path) + ".wxbin"; if (wx_write_bin(ctx, fn.c_str(), 1) < 0) { printf("write_bin %s\n", wx_last_error()); return 1; } }

@@ -17,8 +17,9 @@ Fixtures (numpy .npz, no pickles):
                         list after admesh in slot order (b<s>), the state
                         summaries of the dumped steps in that order.
   hkeys.npz             getHilbertKey of every block ID of two meshes.
-  io.npz                the reference's checkpoint files (writeBin: SHA-256
-                        and size; writeState: text) at a few steps.
+  io.npz                the reference's output files at a few steps:
+                        writeBin and the VTK files (SHA-256 and size),
+                        writeState's text and master.visit.
   unit_<build>.npz      reference routine outputs on seeded inputs
                         (flow2prim, prim2flow, prim2fluxes, limiter, Riemann
                         solvers, wave speeds, find_coef, calcTemp).
@@ -139,7 +140,7 @@ def make_io():
     out = {}
     for name, (n, dumps) in IO_RUNS.items():
         w = W.get(name)
-        r = refrun.run(w, n, dumps, bindump=True)
+        r = refrun.run(w, n, dumps, bindump=True, vtkdump=True)
         out["%s_steps" % name] = r["steps"]
         out["%s_dumps" % name] = np.asarray(dumps, dtype=np.int32)
         for d in dumps:
@@ -147,6 +148,10 @@ def make_io():
             out["%s_d%d_sha256" % (name, d)] = np.asarray(hashlib.sha256(b).hexdigest())
             out["%s_d%d_size" % (name, d)] = np.asarray(len(b), dtype=np.int64)
             out["%s_d%d_state" % (name, d)] = np.asarray(t)
+            for kind, data in zip(("vtk", "grid"), r["vtk"][d]):
+                out["%s_d%d_%s_sha256" % (name, d, kind)] = np.asarray(hashlib.sha256(data).hexdigest())
+                out["%s_d%d_%s_size" % (name, d, kind)] = np.asarray(len(data), dtype=np.int64)
+        out["%s_visit" % name] = np.asarray(r["visit"])
     np.savez_compressed(os.path.join(HERE, "io.npz"), **out)
     print("io %s" % list(IO_RUNS), flush=True)
 

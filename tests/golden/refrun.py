@@ -62,7 +62,7 @@ def read_ints(path):
 
 
 def run(w, nsteps, dumps, workdir=None, unit_op=None, unit_in=None, timeout=3600, tstop=-1.0, variant="",
-        nprocs=1, amr=False, bindump=False):
+        nprocs=1, amr=False, bindump=False, vtkdump=False):
     """Run the harness; returns dict with topo, steps (it, dt, time, tout),
     and states {step: (U, PRIM)} with the passive scalar stripped.  With
     nprocs > 1 the multi-rank build runs under mpiexec and only the step log
@@ -71,7 +71,9 @@ def run(w, nsteps, dumps, workdir=None, unit_op=None, unit_in=None, timeout=3600
     (markByPhysical before admesh) and blocks {step: bids after admesh, in
     the order of that step's state dump}.  With bindump the reference's
     own checkpoint files of each dumped step come back as
-    files {step: (Blocks000.NNNN.bin bytes, State.NNNN.dat text)}."""
+    files {step: (Blocks000.NNNN.bin bytes, State.NNNN.dat text)}; with
+    vtkdump vtk {step: (Blocks000.NNNN.vtk bytes, Grid.NNNN.vtk bytes)} and
+    visit (master.visit text at the end)."""
     if nprocs > 1:
         variant = variant + "_np%d" % nprocs
     if amr:
@@ -101,7 +103,8 @@ def run(w, nsteps, dumps, workdir=None, unit_op=None, unit_in=None, timeout=3600
     with open(os.path.join(wd, "harness.nml"), "w") as f:
         f.write("&ctl\n mode='run'\n nsteps=%d\n ndump=%d\n dumps=%s\n tstop=%r\n%s/\n"
                 % (nsteps, len(dumps), dl or "-1", float(tstop),
-                   (" amr=.true.\n" if amr else "") + (" bindump=.true.\n" if bindump else "")))
+                   (" amr=.true.\n" if amr else "") + (" bindump=.true.\n" if bindump else "")
+                   + (" vtkdump=.true.\n" if vtkdump else "")))
     cmd = [exe] if nprocs == 1 else [MPIEXEC, "-n", str(nprocs), exe]
     r = subprocess.run(cmd, cwd=wd, timeout=timeout, capture_output=True, text=True)
     if r.returncode != 0:
@@ -138,6 +141,16 @@ def run(w, nsteps, dumps, workdir=None, unit_op=None, unit_in=None, timeout=3600
             with open(os.path.join(wd, "State.%04d.dat" % d)) as f:
                 files[d] = (b, f.read())
         out["files"] = files
+    if vtkdump:
+        vtk = {}
+        for d in dumps:
+            with open(os.path.join(wd, "Blocks000.%04d.vtk" % d), "rb") as f:
+                b = f.read()
+            with open(os.path.join(wd, "Grid.%04d.vtk" % d), "rb") as f:
+                vtk[d] = (b, f.read())
+        out["vtk"] = vtk
+        with open(os.path.join(wd, "master.visit")) as f:
+            out["visit"] = f.read()
     if amr:
         out.update(flags=flags, blocks=blocks_after)
     if own:

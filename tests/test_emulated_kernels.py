@@ -80,8 +80,8 @@ def emu():
     return _build()
 
 
-def _run(emu, args, tmp):
-    env = dict(os.environ, ASAN_OPTIONS="detect_stack_use_after_return=0:detect_leaks=0")
+def _run(emu, args, tmp, **env_extra):
+    env = dict(os.environ, ASAN_OPTIONS="detect_stack_use_after_return=0:detect_leaks=0", **env_extra)
     r = subprocess.run([emu] + args, capture_output=True, text=True, timeout=900, env=env, cwd=tmp)
     assert r.returncode == 0 and "ERROR: AddressSanitizer" not in r.stderr, (r.stdout + r.stderr)[-4000:]
     return r.stdout
@@ -109,7 +109,9 @@ def test_emulation_reproduces_fused_fp64(emu, tmp_path, oracle):
     w = W.get("ot64")
     p = str(tmp_path / "ot64.bin")
     blocks, U0 = _write_input(p, w, hip.FUSED_FP64)
-    _run(emu, [p, "3"], str(tmp_path))
+    vtkdir = tmp_path / "vtk"
+    vtkdir.mkdir()
+    _run(emu, [p, "3"], str(tmp_path), EMU_VTK_DIR=str(vtkdir))
     o = oracle.Sim(w, blocks)
     o.set_state(U0)
     for _ in range(3):
@@ -127,6 +129,14 @@ def test_emulation_reproduces_fused_fp64(emu, tmp_path, oracle):
     assert hashlib.sha256(data).hexdigest() == str(fx["ot64_d3_sha256"])
     bids, ur = hip.read_bin(p + ".wxbin", 8, npassive=1)
     assert np.array_equal(bids, np.asarray(blocks)) and np.array_equal(ur, uo)
+    # wx_write_vtk: the reference's VTK files of outputs 0 and 3 and its
+    # master.visit, byte for byte
+    for d in (0, 3):
+        for kind, name in (("vtk", "Blocks000.%04d.vtk" % d), ("grid", "Grid.%04d.vtk" % d)):
+            data = (vtkdir / name).read_bytes()
+            assert len(data) == int(fx["ot64_d%d_%s_size" % (d, kind)]), name
+            assert hashlib.sha256(data).hexdigest() == str(fx["ot64_d%d_%s_sha256" % (d, kind)]), name
+    assert (vtkdir / "master.visit").read_text() == str(fx["ot64_visit"])
 
 
 def test_emulated_admesh_reproduces_the_oracle(emu, tmp_path, oracle):

@@ -1139,6 +1139,62 @@ int wx_read_state(const char *path, double t_sc, double *time, int *it, int *nex
   return WX_OK;
 }
 
+int wx_write_vtk(void *ctx, const char *datadir, int noutput, int npassive, int units) {
+  Ctx *c = as_ctx(ctx);
+  int rc = need_state(c);
+  if (rc != WX_OK) return rc;
+  if (!datadir || npassive < 0 || noutput < 0 || noutput > 9999 || (units != 1 && units != 2))
+    return fail(WX_EINVAL, "bad argument");
+  const int nv = c->cfg.nv;
+  std::vector<double> u((size_t)c->nblk * nv * NINT), prim(u.size());
+  rc = wx_download_state(ctx, u.data(), nullptr, WX_LAYOUT_BLOCKS, 0, nullptr);
+  if (rc < 0) return rc;
+  // writeOutput calls updatePrims before the VTK files: PRIM from U
+  for (int b = 0; b < c->nblk; b++)
+    for (int q = 0; q < NINT; q++) {
+      double uu[MAXV], pp[MAXV];
+      for (int v = 0; v < nv; v++) uu[v] = u[((size_t)b * nv + v) * NINT + q];
+      flow2prim(c->dc.pd, uu, pp);
+      for (int v = 0; v < nv; v++) prim[((size_t)b * nv + v) * NINT + q] = pp[v];
+    }
+  std::vector<int> levels(c->nblk), coords(3 * (size_t)c->nblk);
+  for (int b = 0; b < c->nblk; b++) {
+    levels[b] = c->info[b].level;
+    for (int a = 0; a < 3; a++) coords[3 * b + a] = c->info[b].coords[a];
+  }
+  VtkInfo m;
+  m.nblk = c->nblk;
+  m.nv = nv;
+  m.npassive = npassive;
+  m.maxlev = c->maxlev;
+  m.eos = c->cfg.eos;
+  m.phys_units = units == 1;
+  m.bids = c->bids.data();
+  m.levels = levels.data();
+  m.coords = coords.data();
+  m.dxyz = c->dxyz.data();
+  for (int d = 0; d < 3; d++) m.nroot[d] = c->nroot[d];
+  m.l_sc = c->cfg.l_sc;
+  m.d_sc = c->cfg.d_sc;
+  m.v_sc = c->cfg.v_sc;
+  m.p_sc = c->cfg.p_sc;
+  m.b_sc = std::sqrt(4.0 * PI_REF * c->cfg.d_sc) * c->cfg.v_sc;   // parameters.f90: b_sc
+  m.temp_mu0 = c->dc.temp_mu0;
+  m.temp_mui = c->dc.temp_mui;
+  m.ion_thres = c->cfg.ion_thres;
+  std::string dir(datadir);
+  if (dir.empty() || dir.back() != '/') dir += '/';
+  char name[64];
+  std::snprintf(name, sizeof name, "Blocks000.%04d.vtk", noutput);
+  std::string err;
+  if (visit_write((dir + "master.visit").c_str(), name, noutput, noutput == 0, err) ||
+      vtk_blocks_write((dir + name).c_str(), m, prim.data(), err))
+    return fail(WX_EIO, err);
+  std::snprintf(name, sizeof name, "Grid.%04d.vtk", noutput);
+  if (vtk_grid_write((dir + name).c_str(), m, err)) return fail(WX_EIO, err);
+  return WX_OK;
+}
+
 int wx_record_ghosts(void *ctx) {
   Ctx *c = as_ctx(ctx);
   int rc = need_state(c);

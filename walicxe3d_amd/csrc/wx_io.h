@@ -25,4 +25,28 @@ int state_write(const char *path, double time_phys, int it, int noutput, const c
 int state_read(const char *path, double *time_phys, int *it, int *noutput, char *datadir, int len,
                std::string &err);
 
+// The mesh and scales the VTK writers need.
+struct VtkInfo {
+  int nblk = 0, nv = 5, npassive = 0, maxlev = 1, eos = 0;
+  int phys_units = 0;           // units_type == PHYS_UNITS
+  const int *bids = nullptr;    // [nblk]
+  const int *levels = nullptr;  // [nblk]
+  const int *coords = nullptr;  // [nblk][3], 1-based at the block's level
+  const double *dxyz = nullptr; // dx(1..maxlev), dy(...), dz(...)
+  int nroot[3] = {1, 1, 1};
+  double l_sc = 1, d_sc = 1, v_sc = 1, p_sc = 1, b_sc = 1;
+  double temp_mu0 = 1, temp_mui = 1, ion_thres = 0;   // calcTemp
+};
+
+// write3DVTKBlocks (output.f90:273-750): every cell a hexahedron of its own
+// 8 points; rho, vel, P, B (8 variables), T and the passive scalars (zero:
+// not carried) as FP32, big-endian; prim [blk][nv][16^3].
+int vtk_blocks_write(const char *path, const VtkInfo &m, const double *prim, std::string &err);
+// write3DVTKGrid (output.f90:755-895): one hexahedron per block with its
+// rank (0), bID and Hilbert key.
+int vtk_grid_write(const char *path, const VtkInfo &m, std::string &err);
+// master.visit of write3DVTKBlocks (output.f90:305-340), one rank:
+// replaced at output 0, appended to afterwards.
+int visit_write(const char *path, const char *blocks_name, int noutput, bool replace, std::string &err);
+
 }  // namespace wx

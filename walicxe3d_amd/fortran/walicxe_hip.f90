@@ -121,6 +121,13 @@ module walicxe_hip
       integer(c_int), value :: len
     end function
 
+    integer(c_int) function wx_write_vtk(ctx, datadir, noutput, npassive, units) bind(C, name="wx_write_vtk")
+      import :: c_int, c_ptr, c_char
+      type(c_ptr), value :: ctx
+      character(kind=c_char), intent(in) :: datadir(*)
+      integer(c_int), value :: noutput, npassive, units
+    end function
+
     integer(c_int) function wx_record_ghosts(ctx) bind(C, name="wx_record_ghosts")
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx

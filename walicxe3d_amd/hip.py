@@ -75,6 +75,7 @@ SIGNATURES = {
     "wx_read_bin": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_int, _ip, _ip, _dp]),
     "wx_write_state": (C.c_int, [C.c_char_p, C.c_double, C.c_double, C.c_int, C.c_int, C.c_char_p]),
     "wx_read_state": (C.c_int, [C.c_char_p, C.c_double, _dp, _ip, _ip, C.c_char_p, C.c_int]),
+    "wx_write_vtk": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int, C.c_int, C.c_int]),
 }
 
 FLAG_NONE, FLAG_REFINE, FLAG_COARSE = 0, 1, 2
@@ -245,6 +246,13 @@ class Walicxe3D:
     def writeBin(self, path, npassive=0):
         """The reference's blocks file of the current state (wx_write_bin)."""
         return _check(load().wx_write_bin(self._h, os.fsencode(path), npassive), "wx_write_bin")
+
+    def write_vtk(self, datadir, noutput, npassive=0, units=None):
+        """writeOutput's VTK files of the current state (wx_write_vtk);
+        units "phys" or "code" (default: the workload's)."""
+        units = units or self.w.units
+        return _check(load().wx_write_vtk(self._h, os.fsencode(datadir), noutput, npassive,
+                                          1 if units == "phys" else 2), "wx_write_vtk")
 
     def record_ghosts(self):
         """Fused form: keep the ghost layer of the current state for the next

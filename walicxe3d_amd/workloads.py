@@ -207,6 +207,7 @@ class Workload:
     refine_rule: Optional[str] = None
     refine_radii: Tuple[float, ...] = ()
     winds: Tuple[Tuple[float, ...], ...] = ()
+    units: str = "code"          # units of the VTK output (units_type): "code" or "phys"
     oracle: bool = True
     description: str = ""
 
@@ -305,7 +306,7 @@ _reg(_w(name="mhd_hlle", mhd=True, solver="hlle", limiter="superbee", gamma_expr
 _reg(_w(name="amr_small", mhd=False, solver="hllc", limiter="woodward", gamma_expr="5.0/3.0",
         gamma=5.0 / 3.0, cfl=0.4, visc_eta=5.0e-3, eight_wave=False, bc=FREE6,
         nroot=(1, 1, 1), maxlev=4, base_level=2, phys=(20.0 * PC, 20.0 * PC, 20.0 * PC), ic="snr", seed=8,
-        nsteps=8, cooling=True, refine_rule="snr", **_PHYS_UNITS,
+        nsteps=8, cooling=True, refine_rule="snr", units="phys", **_PHYS_UNITS,
         description="small SNR on 3 AMR levels with radiative cooling"))
 # Mesh adaptation coverage (admesh every step): an off-centre blast in a
 # near-uniform medium, the initial refinement centred elsewhere, so that the
