@@ -131,3 +131,43 @@ def test_admesh_over_capacity_leaves_the_state():
     g.bids, g.nblk = b0, len(b0)
     g.run(1)                                         # the context still steps
     g.close()
+
+
+@pytest.mark.parametrize("form", ["fused_fp32", "plain"])
+def test_mhd_flags_match_oracle(oracle, form):
+    """markByPhysical on an MHD state (pressure recovered with the magnetic
+    term): the device gradients and flags equal the oracle's on the same U,
+    with thresholds inside the gradient distribution so that every flag
+    occurs."""
+    import dataclasses
+    from walicxe3d_amd import hip
+    w3 = W.get("ot64")
+    bids = np.asarray(w3.blocks(), dtype=np.int32)
+    # the same level-3 blocks in a 4-level numbering (same bIDs and spacing),
+    # so that they may be flagged for refinement
+    w = dataclasses.replace(w3, maxlev=4)
+    say("case mhd flags", form)
+    g = hip.Walicxe3D(w, form=hip.FUSED_FP32 if form == "fused_fp32" else hip.PLAIN_FP64, bids=bids)
+    g.upload(ic_in_order(w3, bids))
+    for _ in range(3):
+        g.step()
+    u = g.download(prim=False)
+    _, grad = g.markByPhysical(1e300, 0.0)
+    rt, ct = float(np.percentile(grad, 70)), float(np.percentile(grad, 30))
+    f, grad2 = g.markByPhysical(rt, ct)
+    assert np.array_equal(grad, grad2)
+    o = oracle.Sim(w, bids)
+    o.set_state(u)
+    if form == "plain":
+        # the plain form's ghost layer is the step's first exchange: compare
+        # with the plain PRIM itself
+        o.array(2)[:] = g.raw(2)
+    else:
+        o.boundary(1, 0)
+        o.calc_prims(0, 2)
+    fo, go = o.mark(rt, ct)
+    assert {0, 1, 2} <= set(f.tolist())
+    assert np.array_equal(f, fo)
+    full = go < rt
+    assert np.array_equal(grad[full], go[full])
+    g.close()
