@@ -189,6 +189,12 @@ contains
       write(logu,'(a)') "SOLVER_LAX does not run on the GPU"
       call clean_abort (ERROR_GENERIC)
     end if
+    if (user_source_terms) then
+      ! get_user_source_terms (source/sources.f90:124) is problem code the
+      ! device update does not call
+      write(logu,'(a)') "user_source_terms: get_user_source_terms does not run on the GPU"
+      call clean_abort (ERROR_GENERIC)
+    end if
     call ensure_context()
     call to_device()
 
