@@ -36,6 +36,24 @@ int fail(int code, const std::string &msg) {
     if (e_ != hipSuccess) return fail(WX_EHIP, std::string(#expr ": ") + hipGetErrorString(e_)); \
   } while (0)
 
+// Device scratch that is freed on every return path.
+template <typename T>
+struct DevBuf {
+  T *p = nullptr;
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  ~DevBuf() {
+    if (p) hipFree(p);
+  }
+  hipError_t alloc(size_t n) { return hipMalloc(&p, n * sizeof(T)); }
+  T *release() {
+    T *q = p;
+    p = nullptr;
+    return q;
+  }
+};
+
 struct Ctx {
   wx_config cfg{};
   std::vector<double> tabT, tabC, winds;
@@ -595,18 +613,16 @@ static int check_mesh(const Ctx *c, const std::vector<int> &bids) {
 // (or reads the layer wx_record_ghosts kept).
 static int mark_blocks(Ctx *c, double refine_thres, double coarse_thres, int *flags, double *maxgrad) {
   std::vector<double> m(c->nblk);
-  double *d_m = nullptr;
-  HIPCHK(hipMalloc(&d_m, m.size() * sizeof(double)));
+  DevBuf<double> d_m;
+  HIPCHK(d_m.alloc(m.size()));
   if (is_fused(c))
-    fused_mark(c->dc, c->fused.U, c->ghosts_recorded ? c->d_ghosts : nullptr, d_m, c->st);
+    fused_mark(c->dc, c->fused.U, c->ghosts_recorded ? c->d_ghosts : nullptr, d_m.p, c->st);
   else
-    plain_mark(c->dc, c->dP, d_m, c->st);
+    plain_mark(c->dc, c->dP, d_m.p, c->st);
   int rc = launch_check("mark");
-  if (rc == WX_OK && hipMemcpyAsync(m.data(), d_m, m.size() * sizeof(double), hipMemcpyDeviceToHost, c->st) != hipSuccess)
-    rc = fail(WX_EHIP, "mark read-back failed");
-  if (rc == WX_OK && hipStreamSynchronize(c->st) != hipSuccess) rc = fail(WX_EHIP, "mark failed");
-  hipFree(d_m);
   if (rc != WX_OK) return rc;
+  HIPCHK(hipMemcpyAsync(m.data(), d_m.p, m.size() * sizeof(double), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
   for (int n = 0; n < c->nblk; n++) {
     const int lev = c->info[n].level;
     if (m[n] >= refine_thres)
@@ -764,12 +780,10 @@ int wx_upload_state(void *ctx, const double *u, int layout, int nbmax, const int
   } else if (layout != WX_LAYOUT_BLOCKS) {
     return fail(WX_EINVAL, "unknown layout");
   }
-  double *d_in = nullptr;
-  HIPCHK(hipMalloc(&d_in, ni * sizeof(double)));
-  HIPCHK(hipMemcpyAsync(d_in, src, ni * sizeof(double), hipMemcpyHostToDevice, c->st));
-  int rc = upload_device(c, d_in);
-  hipFree(d_in);
-  return rc;
+  DevBuf<double> d_in;
+  HIPCHK(d_in.alloc(ni));
+  HIPCHK(hipMemcpyAsync(d_in.p, src, ni * sizeof(double), hipMemcpyHostToDevice, c->st));
+  return upload_device(c, d_in.p);
 }
 
 int wx_upload_prims(void *ctx, const double *p, int layout, int nbmax, const int *lb) {
@@ -790,14 +804,14 @@ int wx_upload_prims(void *ctx, const double *p, int layout, int nbmax, const int
   } else if (layout != WX_LAYOUT_BLOCKS) {
     return fail(WX_EINVAL, "unknown layout");
   }
-  double *d_in = nullptr;
-  HIPCHK(hipMalloc(&d_in, ni * sizeof(double)));
-  HIPCHK(hipMemcpyAsync(d_in, src, ni * sizeof(double), hipMemcpyHostToDevice, c->st));
+  DevBuf<double> d_in;
+  HIPCHK(d_in.alloc(ni));
+  HIPCHK(hipMemcpyAsync(d_in.p, src, ni * sizeof(double), hipMemcpyHostToDevice, c->st));
   size_t total = (size_t)c->nblk * nv * NCELL;
-  hipLaunchKernelGGL(k_scatter, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, c->st, nv, c->nblk, d_in, c->dP, 0);
+  const double *in = d_in.p;
+  hipLaunchKernelGGL(k_scatter, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, c->st, nv, c->nblk, in, c->dP, 0);
   rc = launch_check("upload_prims");
   HIPCHK(hipStreamSynchronize(c->st));
-  hipFree(d_in);
   return rc;
 }
 
@@ -807,29 +821,29 @@ int wx_download_state(void *ctx, double *u, double *prim, int layout, int nbmax,
   if (rc != WX_OK) return rc;
   int nv = c->cfg.nv;
   size_t ni = (size_t)c->nblk * nv * NINT;
-  double *d_out = nullptr;
-  HIPCHK(hipMalloc(&d_out, ni * sizeof(double)));
+  DevBuf<double> d_out;
+  HIPCHK(d_out.alloc(ni));
+  double *out = d_out.p;
   std::vector<double> tmp(layout == WX_LAYOUT_FORTRAN ? ni : 0);
   for (int w = 0; w < 2; w++) {
     double *dst = w == 0 ? u : prim;
     if (!dst) continue;
-    const double *from = d_out;
+    const double *from = out;
     if (is_fused(c)) {
       // the fused state is already in the block layout of the interface
       if (w == 0)
         from = c->fused.U;
-      else if (fused_prims(c->fused, c->dc, d_out, c->st))
+      else if (fused_prims(c->fused, c->dc, out, c->st))
         return fail(WX_EHIP, fused_error());
     } else {
       hipLaunchKernelGGL(k_gather, dim3((unsigned)((ni + 255) / 256)), dim3(256), 0, c->st, nv, c->nblk,
-                         w == 0 ? c->dU : c->dP, d_out);
+                         w == 0 ? c->dU : c->dP, out);
     }
     double *h = layout == WX_LAYOUT_FORTRAN ? tmp.data() : dst;
     HIPCHK(hipMemcpyAsync(h, from, ni * sizeof(double), hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
     if (layout == WX_LAYOUT_FORTRAN) to_fortran_or_back(c, tmp.data(), dst, false, nbmax, lb);
   }
-  hipFree(d_out);
   return launch_check("download");
 }
 
@@ -1009,60 +1023,56 @@ int wx_remesh(void *ctx, int nblk, const int *bids) {
   if (is_fused(c)) {
     // the fused state has the block layout k_remap reads and writes: the new
     // state is remapped straight from it and becomes the new fused state
-    double *d_new = nullptr;
-    RemapSrc *d_map = nullptr;
-    HIPCHK(hipMalloc(&d_new, nnew * sizeof(double)));
-    HIPCHK(hipMalloc(&d_map, table.size() * sizeof(RemapSrc)));
-    HIPCHK(hipMemcpyAsync(d_map, table.data(), table.size() * sizeof(RemapSrc), hipMemcpyHostToDevice, c->st));
-    hipLaunchKernelGGL(k_remap, dim3(nblk, nv), dim3(256), 0, c->st, nv, d_map, c->fused.U, d_new);
+    DevBuf<double> d_new;
+    DevBuf<RemapSrc> d_map;
+    HIPCHK(d_new.alloc(nnew));
+    HIPCHK(d_map.alloc(table.size()));
+    HIPCHK(hipMemcpyAsync(d_map.p, table.data(), table.size() * sizeof(RemapSrc), hipMemcpyHostToDevice, c->st));
+    const RemapSrc *map = d_map.p;
+    double *to = d_new.p;
+    hipLaunchKernelGGL(k_remap, dim3(nblk, nv), dim3(256), 0, c->st, nv, map, c->fused.U, to);
     rc = launch_check("remesh");
     HIPCHK(hipStreamSynchronize(c->st));
-    hipFree(d_map);
     if (rc == WX_OK) {
       const std::vector<double> dxyz = c->dxyz;
       int nroot[3] = {c->nroot[0], c->nroot[1], c->nroot[2]};
       rc = wx_set_mesh(ctx, nroot, c->maxlev, dxyz.data(), nblk, bids);
     }
-    if (rc != WX_OK) {
-      hipFree(d_new);
-      return rc;
-    }
-    return upload_device(c, d_new, true);
+    if (rc != WX_OK) return rc;
+    return upload_device(c, d_new.release(), true);   // the fused state owns it now
   }
   // the plain form also carries PRIM (its interior pressure is the one
   // cooling left, not the one U gives back)
   const bool carry = true;
-  double *d_old = nullptr, *d_new = nullptr, *d_newp = nullptr;
-  RemapSrc *d_map = nullptr;
-  HIPCHK(hipMalloc(&d_old, nold * sizeof(double)));
-  HIPCHK(hipMalloc(&d_new, nnew * sizeof(double)));
-  if (carry) HIPCHK(hipMalloc(&d_newp, nnew * sizeof(double)));
-  HIPCHK(hipMalloc(&d_map, table.size() * sizeof(RemapSrc)));
-  HIPCHK(hipMemcpyAsync(d_map, table.data(), table.size() * sizeof(RemapSrc), hipMemcpyHostToDevice, c->st));
-  hipLaunchKernelGGL(k_gather, dim3((unsigned)((nold + 255) / 256)), dim3(256), 0, c->st, nv, c->nblk, c->dU, d_old);
-  hipLaunchKernelGGL(k_remap, dim3(nblk, nv), dim3(256), 0, c->st, nv, d_map, d_old, d_new);
+  DevBuf<double> d_old, d_new, d_newp;
+  DevBuf<RemapSrc> d_map;
+  HIPCHK(d_old.alloc(nold));
+  HIPCHK(d_new.alloc(nnew));
+  if (carry) HIPCHK(d_newp.alloc(nnew));
+  HIPCHK(d_map.alloc(table.size()));
+  HIPCHK(hipMemcpyAsync(d_map.p, table.data(), table.size() * sizeof(RemapSrc), hipMemcpyHostToDevice, c->st));
+  const RemapSrc *map = d_map.p;
+  double *old = d_old.p, *unew = d_new.p, *pnew = d_newp.p;
+  hipLaunchKernelGGL(k_gather, dim3((unsigned)((nold + 255) / 256)), dim3(256), 0, c->st, nv, c->nblk, c->dU, old);
+  hipLaunchKernelGGL(k_remap, dim3(nblk, nv), dim3(256), 0, c->st, nv, map, old, unew);
   if (carry) {
-    hipLaunchKernelGGL(k_gather, dim3((unsigned)((nold + 255) / 256)), dim3(256), 0, c->st, nv, c->nblk, c->dP, d_old);
-    hipLaunchKernelGGL(k_remap, dim3(nblk, nv), dim3(256), 0, c->st, nv, d_map, d_old, d_newp);
+    hipLaunchKernelGGL(k_gather, dim3((unsigned)((nold + 255) / 256)), dim3(256), 0, c->st, nv, c->nblk, c->dP, old);
+    hipLaunchKernelGGL(k_remap, dim3(nblk, nv), dim3(256), 0, c->st, nv, map, old, pnew);
   }
   rc = launch_check("remesh");
   HIPCHK(hipStreamSynchronize(c->st));
-  hipFree(d_old);
   if (rc == WX_OK) {
     const std::vector<double> dxyz = c->dxyz;
     int nroot[3] = {c->nroot[0], c->nroot[1], c->nroot[2]};
     rc = wx_set_mesh(ctx, nroot, c->maxlev, dxyz.data(), nblk, bids);
   }
-  if (rc >= WX_OK) rc = upload_device(c, d_new);
+  if (rc >= WX_OK) rc = upload_device(c, d_new.p);
   if (rc >= WX_OK && carry) {
-    hipLaunchKernelGGL(k_carry_prims, dim3(nblk, nv), dim3(256), 0, c->st, nv, d_map, d_newp, c->dP);
+    hipLaunchKernelGGL(k_carry_prims, dim3(nblk, nv), dim3(256), 0, c->st, nv, map, pnew, c->dP);
     int rc2 = launch_check("remesh prims");
     if (rc2 == WX_OK && hipStreamSynchronize(c->st) != hipSuccess) rc2 = fail(WX_EHIP, "remesh prims failed");
     if (rc2 != WX_OK) rc = rc2;
   }
-  hipFree(d_map);
-  hipFree(d_new);
-  hipFree(d_newp);
   return rc;
 }
 

@@ -199,8 +199,7 @@ static void launch_dt_init(FusedState &f, const DevCfg &c, hipStream_t st) {
 }
 
 int fused_upload(FusedState &f, const DevCfg &c, double *U, bool adopt, hipStream_t st) {
-  if (adopt) {
-    FCHK(hipStreamSynchronize(st));
+  if (adopt) {   // ownership first, so that no return path leaks U
     hipFree(f.U);
     f.U = U;
   } else {
