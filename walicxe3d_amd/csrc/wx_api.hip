@@ -6,11 +6,11 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <string>
 #include <unordered_map>
-#include <unordered_set>
 #include <vector>
 
 #include "wx_device.h"
@@ -74,6 +74,7 @@ struct Ctx {
   bool have_state = false;
   bool ghosts_recorded = false;      // fused form: ghost layer of wx_record_ghosts in d_ghosts
   double *d_ghosts = nullptr;        // [blk][rho, p][6][256]
+  double *d_mark = nullptr;          // [blk] largest gradient (wx_mark_blocks)
   FusedState fused;
 };
 
@@ -88,7 +89,9 @@ void free_mesh(Ctx *c) {
   c->dU = c->dUP = c->dP = c->dF = c->dTmp = nullptr;
   fused_free(c->fused);
   hipFree(c->d_ghosts);
+  hipFree(c->d_mark);
   c->d_ghosts = nullptr;
+  c->d_mark = nullptr;
   c->have_state = false;
   c->ghosts_recorded = false;
 }
@@ -472,6 +475,36 @@ __global__ void __launch_bounds__(256) k_carry_prims(int nv, const RemapSrc *map
 // ---------------------------------------------------------------------------
 enum { FLAG_NONE = 0, FLAG_REFINE = 1, FLAG_COARSE = 2 };   // constants.f90:76-78
 
+// bID -> index with open addressing (linear probing); one allocation per
+// plan, where std::unordered_map allocates per entry
+struct BidMap {
+  std::vector<long> key;
+  std::vector<int> val;
+  size_t mask = 0;
+  explicit BidMap(size_t n) {
+    size_t cap = 16;
+    while (cap < 2 * n) cap <<= 1;
+    key.assign(cap, -1);
+    val.assign(cap, -1);
+    mask = cap - 1;
+  }
+  static size_t hash(long k) { return (size_t)(((uint64_t)k * 0x9E3779B97F4A7C15ull) >> 20); }
+  bool insert(long k, int v) {
+    size_t i = hash(k) & mask;
+    for (; key[i] != -1; i = (i + 1) & mask)
+      if (key[i] == k) return false;
+    key[i] = k;
+    val[i] = v;
+    return true;
+  }
+  int find(long k) const {
+    for (size_t i = hash(k) & mask; key[i] != -1; i = (i + 1) & mask)
+      if (key[i] == k) return val[i];
+    return -1;
+  }
+  bool count(long k) const { return find(k) >= 0; }
+};
+
 static long amr_father(const Numbering &N, long bid) {
   const int lev = N.level(bid);
   if (lev <= 1) return -1;
@@ -494,7 +527,7 @@ static long amr_child(const Numbering &N, long bid, int q) {
 // every domain face (neighborLevel, :439-523, does not wrap periodic faces);
 // for NEIGH_SAME and NEIGH_COARSER *id is the neighbour, for NEIGH_FINER the
 // inactive same-level block whose children face bid.
-static int amr_neighbour(const Numbering &N, const std::unordered_map<long, int> &act, long bid, int dir, long *id) {
+static int amr_neighbour(const Numbering &N, const BidMap &act, long bid, int dir, long *id) {
   const int lev = N.level(bid);
   int c[3];
   N.coords(bid, lev, c);
@@ -525,13 +558,12 @@ static int amr_neighbour(const Numbering &N, const std::unordered_map<long, int>
 // turn COARSE into NONE.
 static int amr_plan(const Numbering &N, const std::vector<int> &bids, std::vector<int> &flag, std::vector<int> &out) {
   const size_t nb = bids.size();
-  std::unordered_map<long, int> act;
-  act.reserve(2 * nb);
+  BidMap act(nb);
   std::vector<std::vector<int>> by_level(N.maxlev + 1);
   for (size_t n = 0; n < nb; n++) {
     const int lev = N.level(bids[n]);
     if (lev < 0) return fail(WX_ETOPO, "block ID outside the mesh: " + std::to_string(bids[n]));
-    if (!act.emplace(bids[n], (int)n).second) return fail(WX_ETOPO, "duplicate block ID " + std::to_string(bids[n]));
+    if (!act.insert(bids[n], (int)n)) return fail(WX_ETOPO, "duplicate block ID " + std::to_string(bids[n]));
     if (flag[n] < FLAG_NONE || flag[n] > FLAG_COARSE) return fail(WX_EINVAL, "unknown refinement flag");
     by_level[lev].push_back((int)n);
   }
@@ -545,10 +577,10 @@ static int amr_plan(const Numbering &N, const std::vector<int> &bids, std::vecto
           long id = -1;
           const int t = amr_neighbour(N, act, b, dir, &id);
           if (t == NEIGH_SAME) {
-            int &f = flag[act.at(id)];
+            int &f = flag[act.find(id)];
             if (f == FLAG_COARSE) f = FLAG_NONE;
           } else if (t == NEIGH_COARSER) {
-            flag[act.at(id)] = FLAG_REFINE;
+            flag[act.find(id)] = FLAG_REFINE;
           }
         }
       } else if (bflag == FLAG_COARSE) {
@@ -558,8 +590,8 @@ static int amr_plan(const Numbering &N, const std::vector<int> &bids, std::vecto
           long id = -1;
           if (amr_neighbour(N, act, b, dir, &id) != NEIGH_FINER) continue;
           for (int q = 0; q < 8 && !inhibited; q++) {
-            auto it = act.find(amr_child(N, id, q));
-            if (it != act.end() && flag[it->second] != FLAG_COARSE) {
+            const int m = act.find(amr_child(N, id, q));
+            if (m >= 0 && flag[m] != FLAG_COARSE) {
               flag[n] = FLAG_NONE;
               inhibited = true;
             }
@@ -567,15 +599,23 @@ static int amr_plan(const Numbering &N, const std::vector<int> &bids, std::vecto
         }
       }
     }
-  std::unordered_map<long, int> family;   // father -> members flagged
+  BidMap family(nb);                       // father -> slot of its count
+  std::vector<int> flagged;                 // members flagged, per family
   for (size_t n = 0; n < nb; n++)
     if (flag[n] == FLAG_COARSE) {
       const long f = amr_father(N, bids[n]);
-      if (f != -1) family[f]++;
+      if (f == -1) continue;
+      int s = family.find(f);
+      if (s < 0) {
+        s = (int)flagged.size();
+        family.insert(f, s);
+        flagged.push_back(0);
+      }
+      flagged[s]++;
     }
   out.clear();
   out.reserve(nb);
-  std::unordered_set<long> placed;
+  std::vector<char> placed(flagged.size(), 0);
   for (size_t n = 0; n < nb; n++) {
     const long b = bids[n];
     if (flag[n] == FLAG_REFINE) {
@@ -586,9 +626,12 @@ static int amr_plan(const Numbering &N, const std::vector<int> &bids, std::vecto
     }
     if (flag[n] == FLAG_COARSE) {
       const long f = amr_father(N, b);
-      auto it = family.find(f);
-      if (f != -1 && it != family.end() && it->second == 8) {
-        if (placed.insert(f).second) out.push_back((int)f);
+      const int s = f == -1 ? -1 : family.find(f);
+      if (s >= 0 && flagged[s] == 8) {
+        if (!placed[s]) {
+          placed[s] = 1;
+          out.push_back((int)f);
+        }
         continue;
       }
     }
@@ -613,15 +656,14 @@ static int check_mesh(const Ctx *c, const std::vector<int> &bids) {
 // (or reads the layer wx_record_ghosts kept).
 static int mark_blocks(Ctx *c, double refine_thres, double coarse_thres, int *flags, double *maxgrad) {
   std::vector<double> m(c->nblk);
-  DevBuf<double> d_m;
-  HIPCHK(d_m.alloc(m.size()));
+  if (!c->d_mark) HIPCHK(hipMalloc(&c->d_mark, m.size() * sizeof(double)));
   if (is_fused(c))
-    fused_mark(c->dc, c->fused.U, c->ghosts_recorded ? c->d_ghosts : nullptr, d_m.p, c->st);
+    fused_mark(c->dc, c->fused.U, c->ghosts_recorded ? c->d_ghosts : nullptr, c->d_mark, c->st);
   else
-    plain_mark(c->dc, c->dP, d_m.p, c->st);
+    plain_mark(c->dc, c->dP, c->d_mark, c->st);
   int rc = launch_check("mark");
   if (rc != WX_OK) return rc;
-  HIPCHK(hipMemcpyAsync(m.data(), d_m.p, m.size() * sizeof(double), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipMemcpyAsync(m.data(), c->d_mark, m.size() * sizeof(double), hipMemcpyDeviceToHost, c->st));
   HIPCHK(hipStreamSynchronize(c->st));
   for (int n = 0; n < c->nblk; n++) {
     const int lev = c->info[n].level;
