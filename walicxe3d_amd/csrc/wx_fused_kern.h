@@ -49,6 +49,15 @@ __device__ WX_RIEMANN_INLINE void riemann_t(const Phys<RF> &P, const RF *pl, con
   else primfhll(P, pl, pr, ff);
 }
 
+// limiter of one variable: fixed at compile time (minmod in FP32 as a
+// median) or read at run time
+constexpr int LIM_RUNTIME = -100;
+template <int LIM, typename RF>
+__device__ __forceinline__ void limit_t(const Phys<RF> &P, RF pll, RF &pl, RF &pr, RF prr) {
+  if constexpr (LIM == 1 && sizeof(RF) == 4) limit1_minmod32(pll, pl, pr, prr);
+  else limit1(pll, pl, pr, prr, LIM == LIM_RUNTIME ? P.limiter : LIM);
+}
+
 // exchange of the x components with those along d (0 x, 1 y, 2 z) of
 // velocity and field, written with selects so that the arrays stay in
 // registers when d is not a compile-time constant (swapxy/swapxz,
@@ -248,7 +257,6 @@ __device__ __forceinline__ void finish_cell(const FusedArgs &a, int b, int i, in
 // Launch bound: 256 threads, at least three waves per SIMD (four for the MHD
 // first stage).
 // ---------------------------------------------------------------------------
-constexpr int LIM_RUNTIME = -100;
 #ifndef WX_STAGE_MIN_WAVES
 #define WX_STAGE_MIN_WAVES 3
 #endif
@@ -347,7 +355,7 @@ __global__ void __launch_bounds__(256, (NV == 8 && !S2) ? WX_MHD_STAGE1_MIN_WAVE
           rot<NV>(2, pll);
           rot<NV>(2, prr);
 #pragma unroll
-          for (int v = 0; v < NV; v++) limit1(pll[v], pl[v], pr[v], prr[v], LIM == LIM_RUNTIME ? P.limiter : LIM);
+          for (int v = 0; v < NV; v++) limit_t<LIM>(P, pll[v], pl[v], pr[v], prr[v]);
         }
         riemann_t<SOLVER>(P, pl, pr, fzc);
         rot<NV>(2, fzc);
@@ -405,7 +413,7 @@ __global__ void __launch_bounds__(256, (NV == 8 && !S2) ? WX_MHD_STAGE1_MIN_WAVE
           rot<NV>(d, pll);
           rot<NV>(d, prr);
 #pragma unroll
-          for (int v = 0; v < NV; v++) limit1(pll[v], pl[v], pr[v], prr[v], LIM == LIM_RUNTIME ? P.limiter : LIM);
+          for (int v = 0; v < NV; v++) limit_t<LIM>(P, pll[v], pl[v], pr[v], prr[v]);
         }
         riemann_t<SOLVER>(P, pl, pr, ff);
         rot<NV>(d, ff);

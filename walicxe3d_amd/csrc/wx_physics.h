@@ -44,6 +44,18 @@ __host__ __device__ inline float rcp32(float b) {
   return 1.0f / b;
 #endif
 }
+// Square root in the MHD wave-speed and HLLD code.  FP64: IEEE square
+// root.  FP32 on the device: the hardware square root (about 1 ulp), without
+// the range scaling that keeps the library's sqrtf exact for arguments
+// below 2^-96 (wave speeds and densities of a flow are far above that).
+__host__ __device__ inline double msqrt(double a) { return sqrt(a); }
+__host__ __device__ inline float msqrt(float a) {
+#ifdef __HIP_DEVICE_COMPILE__
+  return __builtin_amdgcn_sqrtf(a);
+#else
+  return sqrtf(a);
+#endif
+}
 __host__ __device__ inline double rdiv(double a, double b) { return a / b; }
 __host__ __device__ inline float rdiv(float a, float b) {
 #ifdef __HIP_DEVICE_COMPILE__
@@ -128,15 +140,15 @@ template <typename R>
 __host__ __device__ inline R cfast1(const Phys<R> &c, R p, R d, R bx, R by, R bz, R bd) {
   R b2 = bx * bx + by * by + bz * bz;
   R a = c.gamma * p + b2;
-  return rsqrt_(rdiv(R(0.5) * (a + rsqrt_(a * a - R(4.0) * c.gamma * p * bd * bd)), d));
+  return msqrt(rdiv(R(0.5) * (a + msqrt(a * a - R(4.0) * c.gamma * p * bd * bd)), d));
 }
 
 // cfastX (source/hydro_core.f90:553-566)
 template <typename R> __host__ __device__ inline R cfastX(const Phys<R> &c, const R *p) {
   R b2 = p[5] * p[5] + p[6] * p[6] + p[7] * p[7];
   R cs2va2 = rdiv(c.gamma * p[4] + b2, p[0]);
-  return rsqrt_(R(0.5) *
-                (cs2va2 + rsqrt_(cs2va2 * cs2va2 - rdiv(rdiv(R(4.0) * c.gamma * p[4] * (p[5] * p[5]), p[0]), p[0]))));
+  return msqrt(R(0.5) *
+               (cs2va2 + msqrt(cs2va2 * cs2va2 - rdiv(rdiv(R(4.0) * c.gamma * p[4] * (p[5] * p[5]), p[0]), p[0]))));
 }
 
 // average (source/hydro_core.f90:640-697)
@@ -182,6 +194,25 @@ __host__ __device__ inline void limit1(R pll, R &pl, R &pr, R prr, int lim) {
   R al = average(dl, dm, lim), ar = average(dm, dr, lim);
   pl = pl + R(0.5) * al;
   pr = pr - R(0.5) * ar;
+}
+
+// limit1 with the minmod limiter in FP32: minmod(a, b) is the median of a, b
+// and 0 (the smaller for a, b > 0, the larger for a, b < 0, zero when the
+// signs differ), one v_med3_f32 on the device.  The value equals the
+// reference's s*max(0, min(|a|, s*b)) (source/hydro_core.f90:640-697,
+// case 1) except for the sign of a zero slope.
+__host__ __device__ inline float minmod32(float a, float b) {
+#ifdef __HIP_DEVICE_COMPILE__
+  return __builtin_amdgcn_fmed3f(a, b, 0.0f);
+#else
+  return fmaxf(fminf(a, b), fminf(fmaxf(a, b), 0.0f));
+#endif
+}
+__host__ __device__ inline void limit1_minmod32(float pll, float &pl, float &pr, float prr) {
+  float dl = pl - pll, dm = pr - pl, dr = prr - pr;
+  float al = minmod32(dl, dm), ar = minmod32(dm, dr);
+  pl = pl + 0.5f * al;
+  pr = pr - 0.5f * ar;
 }
 
 // HLL / HLLE flux (source/hll.f90:175-215, source/hlle.f90:166-206);
@@ -289,8 +320,8 @@ __host__ __device__ inline void primfhlld(const Phys<R> &c, const R *pl, const R
   R slmsM = sl - sM;
   R rhostl = rdiv(pl[0] * slmul, slmsM);
   R rhostr = rdiv(pr[0] * srmur, srmsM);
-  R sstl = sM - rdiv(rabs(bx), rsqrt_(rhostl));
-  R sstr = sM + rdiv(rabs(bx), rsqrt_(rhostr));
+  R sstl = sM - rdiv(rabs(bx), msqrt(rhostl));
+  R sstr = sM + rdiv(rabs(bx), msqrt(rhostr));
   R pst = rdiv(srmur * pr[0] * pTL - slmul * pl[0] * pTR + pl[0] * pr[0] * srmur * slmul * (pr[1] - pl[1]),
                srmur * pr[0] - slmul * pl[0]);
   R vstl, wstl, bystl, bzstl, vstr, wstr, bystr, bzstr;
@@ -330,11 +361,11 @@ __host__ __device__ inline void primfhlld(const Phys<R> &c, const R *pl, const R
   }
   hlld_star(pl, bx, slmul, slmsM, sM - pl[1], vstl, wstl, bystl, bzstl);
   hlld_star(pr, bx, srmur, srmsM, sM - pr[1], vstr, wstr, bystr, bzstr);
-  R sql = rsqrt_(rhostl), sqr = rsqrt_(rhostr);
+  R sql = msqrt(rhostl), sqr = msqrt(rhostr);
   R dd = sql + sqr;
   R vstst = rdiv(sql * vstl + sqr * vstr + (bystr - bystl) * signBx, dd);
   R wstst = rdiv(sql * wstl + sqr * wstr + (bzstr - bzstl) * signBx, dd);
-  R sqlr = rsqrt_(rhostl * rhostr);
+  R sqlr = msqrt(rhostl * rhostr);
   R bystst = rdiv(sql * bystr + sqr * bystl + sqlr * (vstr - vstl) * signBx, dd);
   R bzstst = rdiv(sql * bzstr + sqr * bzstl + sqlr * (wstr - wstl) * signBx, dd);
   R vststdotbstst = sM * bx + vstst * bystst + wstst * bzstst;
@@ -402,8 +433,8 @@ __host__ __device__ inline void primfhlld_sel(const Phys<R> &c, const R *pl, con
   R slmsM = sl - sM;
   R rhostl = rdiv(pl[0] * slmul, slmsM);
   R rhostr = rdiv(pr[0] * srmur, srmsM);
-  R sstl = sM - rdiv(rabs(bx), rsqrt_(rhostl));
-  R sstr = sM + rdiv(rabs(bx), rsqrt_(rhostr));
+  R sstl = sM - rdiv(rabs(bx), msqrt(rhostl));
+  R sstr = sM + rdiv(rabs(bx), msqrt(rhostr));
   R pst = rdiv(srmur * pr[0] * pTL - slmul * pl[0] * pTR + pl[0] * pr[0] * srmur * slmul * (pr[1] - pl[1]),
                srmur * pr[0] - slmul * pl[0]);
   // region, in the order the reference tests it
@@ -436,11 +467,11 @@ __host__ __device__ inline void primfhlld_sel(const Phys<R> &c, const R *pl, con
   R estK = rdiv(sKmuK * eK - pTK * pK[1] + pst * sM + bx * (vdotbK - vstdotbstK), sKmsM);
   R vq = vstK, wq = wstK, byq = bystK, bzq = bzstK, eq = estK, vdbq = vstdotbstK;
   if (!single) {
-    R sql = rsqrt_(rhostl), sqr = rsqrt_(rhostr);
+    R sql = msqrt(rhostl), sqr = msqrt(rhostr);
     R dd = sql + sqr;
     R vstst = rdiv(sql * vstl + sqr * vstr + (bystr - bystl) * signBx, dd);
     R wstst = rdiv(sql * wstl + sqr * wstr + (bzstr - bzstl) * signBx, dd);
-    R sqlr = rsqrt_(rhostl * rhostr);
+    R sqlr = msqrt(rhostl * rhostr);
     R bystst = rdiv(sql * bystr + sqr * bystl + sqlr * (vstr - vstl) * signBx, dd);
     R bzstst = rdiv(sql * bzstr + sqr * bzstl + sqlr * (wstr - wstl) * signBx, dd);
     R vststdotbstst = sM * bx + vstst * bystst + wstst * bzstst;
