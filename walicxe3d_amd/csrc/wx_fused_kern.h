@@ -35,14 +35,16 @@ template <> __device__ __forceinline__ const Phys<double> &phys<double>(const De
 #else
 #define WX_RIEMANN_INLINE __forceinline__
 #endif
-template <int SOLVER, typename RF>
+// (S1 = first stage: the FP32 HLLD there takes its supersonic cases on
+// one code path)
+template <int SOLVER, typename RF, bool S1 = false>
 __device__ WX_RIEMANN_INLINE void riemann_t(const Phys<RF> &P, const RF *pl, const RF *pr, RF *ff) {
   if constexpr (SOLVER == SOLVER_HLLC) primfhllc(P, pl, pr, ff);
   else if constexpr (SOLVER == SOLVER_HLLD) {
 #ifdef WX_HLLD_BRANCHES
     primfhlld(P, pl, pr, ff);
 #else
-    primfhlld_sel(P, pl, pr, ff);
+    primfhlld_sel<RF, S1 && sizeof(RF) == 4>(P, pl, pr, ff);
 #endif
   }
   else if constexpr (SOLVER == SOLVER_HLLE) primfhlle(P, pl, pr, ff);
@@ -357,7 +359,7 @@ __global__ void __launch_bounds__(256, (NV == 8 && !S2) ? WX_MHD_STAGE1_MIN_WAVE
 #pragma unroll
           for (int v = 0; v < NV; v++) limit_t<LIM>(P, pll[v], pl[v], pr[v], prr[v]);
         }
-        riemann_t<SOLVER>(P, pl, pr, fzc);
+        riemann_t<SOLVER, RF, !S2>(P, pl, pr, fzc);
         rot<NV>(2, fzc);
       }
       if (k == 0) {
@@ -415,7 +417,7 @@ __global__ void __launch_bounds__(256, (NV == 8 && !S2) ? WX_MHD_STAGE1_MIN_WAVE
 #pragma unroll
           for (int v = 0; v < NV; v++) limit_t<LIM>(P, pll[v], pl[v], pr[v], prr[v]);
         }
-        riemann_t<SOLVER>(P, pl, pr, ff);
+        riemann_t<SOLVER, RF, !S2>(P, pl, pr, ff);
         rot<NV>(d, ff);
         if (d == 0) {
 #pragma unroll

@@ -412,14 +412,25 @@ __host__ __device__ inline void primfhlld(const Phys<R> &c, const R *pl, const R
 // branch.  Every value is the expression primfhlld uses for its region, so
 // the two functions agree bit for bit; a wavefront whose lanes fall into
 // different regions no longer runs each region's code in turn.
-template <typename R>
+// MERGED: one code path for the two supersonic cases (the flux of the
+// upwind state); the separate paths spill less in the second-stage kernel.
+template <typename R, bool MERGED = false>
 __host__ __device__ inline void primfhlld_sel(const Phys<R> &c, const R *pl, const R *pr, R *ff) {
   const R cv = c.cv;
   R csl = cfastX(c, pl), csr = cfastX(c, pr);
   R sr = rfmax(pl[1] + csl, pr[1] + csr);
   R sl = rfmin(pl[1] - csl, pr[1] - csr);
-  if (sl > R(0)) { xflux(c, pl, ff); return; }
-  if (sr < R(0)) { xflux(c, pr, ff); return; }
+  if constexpr (MERGED) {
+    if (sl > R(0) || sr < R(0)) {
+      R pu[8];
+      for (int q = 0; q < 8; q++) pu[q] = sl > R(0) ? pl[q] : pr[q];
+      xflux(c, pu, ff);
+      return;
+    }
+  } else {
+    if (sl > R(0)) { xflux(c, pl, ff); return; }
+    if (sr < R(0)) { xflux(c, pr, ff); return; }
+  }
   R bx = R(0.5) * (pl[5] + pr[5]);
   R signBx = rcopysign(R(1), bx);
   R pTL = pl[4] + R(0.5) * (bx * bx + pl[6] * pl[6] + pl[7] * pl[7]);
