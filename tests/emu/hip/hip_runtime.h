@@ -101,6 +101,8 @@ double emu_shfl_xor(double v, int off);
 inline double __shfl_xor(double v, int off) { return emu_shfl_xor(v, off); }
 inline long long __double_as_longlong(double d) { long long r; memcpy(&r, &d, 8); return r; }
 inline double __longlong_as_double(long long l) { double r; memcpy(&r, &l, 8); return r; }
+inline int __float_as_int(float f) { int r; memcpy(&r, &f, 4); return r; }
+inline float __int_as_float(int i) { float r; memcpy(&r, &i, 4); return r; }
 inline unsigned long long atomicMin(unsigned long long *p, unsigned long long v) {
   std::atomic_ref<unsigned long long> a(*p);
   unsigned long long o = a.load();

@@ -51,6 +51,23 @@ __device__ WX_RIEMANN_INLINE void riemann_t(const Phys<RF> &P, const RF *pl, con
   else primfhll(P, pl, pr, ff);
 }
 
+// a value that is the same in every lane, read from the first active lane
+// into scalar registers (the value itself off the device)
+__device__ __forceinline__ int uni(int x) {
+#ifdef __HIP_DEVICE_COMPILE__
+  return __builtin_amdgcn_readfirstlane(x);
+#else
+  return x;
+#endif
+}
+__device__ __forceinline__ float uni(float x) { return __int_as_float(uni(__float_as_int(x))); }
+__device__ __forceinline__ double uni(double x) {
+  const long long v = __double_as_longlong(x);
+  const unsigned lo = (unsigned)uni((int)(unsigned)v);
+  const unsigned hi = (unsigned)uni((int)(unsigned)(v >> 32));
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 // limiter of one variable: fixed at compile time (minmod in FP32 as a
 // median) or read at run time
 constexpr int LIM_RUNTIME = -100;
@@ -293,16 +310,23 @@ __global__ void __launch_bounds__(256, (NV == 8 && !S2) ? WX_MHD_STAGE1_MIN_WAVE
   const int tid = threadIdx.x;
   const int ti = tid & 15, tj = tid >> 4;
   const int i = ti + 1, j = tj + 1;
-  const double dt = a.step->dt;
+  const double dt = NV == 8 ? uni(a.step->dt) : a.step->dt;
   const double dtp = S2 ? dt : dt / 2.0;
   double dtmin = 1.0e30;
 
   for (int bi = blockIdx.x; bi < a.nblk; bi += gridDim.x) {
-    const int b = a.order[bi];
-    const int l = c.info[b].level - 1;
+    // the block and its level are the same in every lane: in the MHD kernels
+    // they are read into scalar registers, so that block addresses and
+    // per-level values stay scalar
+    const int b = NV == 8 ? uni(a.order[bi]) : a.order[bi];
+    const int l = NV == 8 ? uni(c.info[b].level - 1) : c.info[b].level - 1;
     const double dxl = c.dx[0][l], dyl = c.dx[1][l], dzl = c.dx[2][l];
-    const double dtdx = dtp / dxl, dtdy = dtp / dyl, dtdz = dtp / dzl;
-    const RF idx2 = (RF)(1.0 / (2.0 * dxl)), idy2 = (RF)(1.0 / (2.0 * dyl)), idz2 = (RF)(1.0 / (2.0 * dzl));
+    double dtdx = dtp / dxl, dtdy = dtp / dyl, dtdz = dtp / dzl;
+    RF idx2 = (RF)(1.0 / (2.0 * dxl)), idy2 = (RF)(1.0 / (2.0 * dyl)), idz2 = (RF)(1.0 / (2.0 * dzl));
+    if constexpr (NV == 8) {
+      dtdx = uni(dtdx); dtdy = uni(dtdy); dtdz = uni(dtdz);
+      idx2 = uni(idx2); idy2 = uni(idy2); idz2 = uni(idz2);
+    }
 
     // column window of primitives; at iteration k (after the shift)
     // S2: win = planes {k-1, k, k+1, k+2};  S1: win = {k-1, k, k+1}
