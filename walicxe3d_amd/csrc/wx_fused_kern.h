@@ -315,9 +315,10 @@ __global__ void __launch_bounds__(256, (NV == 8 && !S2) ? WX_MHD_STAGE1_MIN_WAVE
   double dtmin = 1.0e30;
 
   for (int bi = blockIdx.x; bi < a.nblk; bi += gridDim.x) {
-    // the block and its level are the same in every lane: in the MHD kernels
-    // they are read into scalar registers, so that block addresses and
-    // per-level values stay scalar
+    // the block and its level are the same in every lane (and every lane is
+    // active here, at the top of the block loop): in the MHD kernels they are
+    // read into scalar registers, so that block addresses and per-level
+    // values stay scalar
     const int b = NV == 8 ? uni(a.order[bi]) : a.order[bi];
     const int l = NV == 8 ? uni(c.info[b].level - 1) : c.info[b].level - 1;
     const double dxl = c.dx[0][l], dyl = c.dx[1][l], dzl = c.dx[2][l];
