@@ -84,7 +84,9 @@ const char *wx_version(void);
    dxyz[3*maxlev] (dx(1..maxlev), dy(..), dz(..): basegrid,
    source/init.f90:726-730) and the active block IDs.  Builds the face
    neighbour table that source/amr.f90:539-631 `neighbors` computes per
-   query.  Must be called again whenever the block list changes. */
+   query.  Must be called again whenever the block list changes.  Returns
+   WX_ETOPO when a periodic domain face wraps onto a block that is not
+   active at the same level (the periodic fill copies same-level blocks). */
 int wx_set_mesh(void *ctx, const int nroot[3], int maxlev, const double *dxyz, int nblk,
                 const int *bids);
 

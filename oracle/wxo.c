@@ -536,7 +536,11 @@ static int bid_of(const wxo_sim *s, int l, int x, int y, int z) {
   return (int)(1 + (x - 1) + (long)(y - 1) * nx + (long)(z - 1) * nx * ny + offsets(s, l));
 }
 
-static void build_topology(wxo_sim *s) {
+/* Returns the number of periodic domain faces whose same-level partner is
+   not active: the periodic fill (userBoundary) copies the block across the
+   face at the same level, and the reference's own build of it stops there. */
+static int build_topology(wxo_sim *s) {
+  int unpaired = 0;
   imap m;
   imap_init(&m, s->nblk);
   for (int b = 0; b < s->nblk; b++) imap_put(&m, s->bid[b], b);
@@ -575,6 +579,7 @@ static void build_topology(wxo_sim *s) {
             case BOTTOM: zs = nz; break; case TOP: zs = 1; break;
           }
           s->periodic[b][f - 1] = imap_get(&m, bid_of(s, l, xs, ys, zs));
+          if (s->periodic[b][f - 1] < 0) unpaired++;
         }
         continue;
       }
@@ -595,6 +600,7 @@ static void build_topology(wxo_sim *s) {
     }
   }
   free(m.key); free(m.val);
+  return unpaired;
 }
 
 /* ------------------------------------------------------------------ */
@@ -1040,7 +1046,10 @@ wxo_sim *wxo_create(const wxo_config *cfg, const int nroot[3], int maxlev, const
   s->nidx = malloc(sizeof(*s->nidx) * nblk);
   s->periodic = malloc(sizeof(*s->periodic) * nblk);
   for (int b = 0; b < nblk; b++) { s->bid[b] = bids[b]; s->lev[b] = meshlevel(s, bids[b]); }
-  build_topology(s);
+  if (build_topology(s) != 0) {  /* a periodic face without a same-level partner */
+    wxo_destroy(s);
+    return NULL;
+  }
   size_t n = (size_t)nblk * cfg->nv * NCELL;
   s->U = (double *)calloc(n, sizeof(double));
   s->UP = (double *)calloc(n, sizeof(double));

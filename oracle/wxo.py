@@ -122,6 +122,9 @@ class Sim:
         nroot = (C.c_int * 3)(*m.nroot)
         self._h = lib().wxo_create(C.byref(self.cfg), nroot, m.maxlev, _ptr(dx), len(self.bids),
                                    self.bids.ctypes.data_as(_ip))
+        if not self._h:
+            raise ValueError("a periodic domain face has no active block across it at the same level "
+                             "(the periodic fill copies same-level blocks only)")
         self.nblk = len(self.bids)
         self.it = C.c_int(0)
         self.dt = C.c_double(0.0)

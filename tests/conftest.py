@@ -73,3 +73,24 @@ def rel_l1(a, b):
     d = np.abs(a - b).reshape(a.shape[0], nv, -1).sum(axis=(0, 2))
     s = np.abs(b).reshape(a.shape[0], nv, -1).sum(axis=(0, 2))
     return np.where(s > 0, d / np.maximum(s, 1e-300), d)
+
+
+def mixed_level_ot(bc=None):
+    """Orszag-Tang 64^3 (64 blocks at level 3 of 3) with two families merged
+    into their level-2 fathers, whose state is the 2x2x2 restriction of the
+    initial condition: (workload, block IDs, state).  Every coarse block has
+    finer neighbours across its faces.  `bc` replaces the workload's periodic
+    boundaries (on a periodic domain some of those faces wrap around)."""
+    import dataclasses
+    import numpy as np
+    from walicxe3d_amd import regrid
+    from walicxe3d_amd import workloads as W
+    w = W.get("ot64")
+    if bc is not None:
+        w = dataclasses.replace(w, bc=(bc,) * 6)
+    m = w.mesh()
+    b1 = np.asarray(w.blocks(), dtype=np.int32)
+    fathers = [m.bid(2, 1, 1, 1), m.bid(2, 2, 2, 1)]
+    gone = {c for f in fathers for c in m.children(f)}
+    b2 = np.asarray([b for b in b1.tolist() if b not in gone] + fathers, dtype=np.int32)
+    return w, b2, regrid.remap(m, b1, ic_in_order(w, b1), b2)

@@ -201,3 +201,16 @@ def test_hilbert_curve_steps_between_face_neighbours(maxlev):
         x, y, z = m.coords(b)
         low = m.bid(maxlev, 2 * x - 1, 2 * y - 1, 2 * z - 1)
         assert hip.hilbert_keys(m, [b])[0] == hip.hilbert_keys(m, [low])[0]
+
+
+def test_oracle_needs_same_level_periodic_partners(oracle):
+    """The periodic fill copies the block across a periodic face at the same
+    level (the reference has no periodic fill of its own,
+    source/boundary.f90:594); a mesh whose periodic faces wrap onto a block
+    of another level is refused, the same mesh with free boundaries is not."""
+    from conftest import mixed_level_ot
+    w, bids, _ = mixed_level_ot()
+    with pytest.raises(ValueError):
+        oracle.Sim(w, bids)
+    w, bids, _ = mixed_level_ot("free")
+    oracle.Sim(w, bids)

@@ -167,9 +167,11 @@ int build_topology(Ctx *c, const int *bids) {
       int n = f >> 1, low = (f & 1) == 0;
       int p[3] = {B.coords[0], B.coords[1], B.coords[2]};
       p[n] += low ? -1 : 1;
+      bool wrapped = false;
       if (p[n] < 1 || p[n] > nbl[n]) {
         if (c->cfg.bc[f] != BC_PERIODIC) continue;
         p[n] = low ? (int)nbl[n] : 1;  // periodic wrap
+        wrapped = true;
       }
       long nid = N.bid(B.level, p);
       auto it = c->index.find((int)nid);
@@ -178,6 +180,13 @@ int build_topology(Ctx *c, const int *bids) {
         F.src[0] = it->second;
         continue;
       }
+      // the reference fills no periodic ghosts (source/boundary.f90:594); the
+      // periodic fill it needs copies the block across the face at the same
+      // level, and has nothing to copy from when that block is refined or
+      // coarsened
+      if (wrapped)
+        return fail(WX_ETOPO, "periodic face " + std::to_string(f) + " of block " + std::to_string(B.bid) +
+                                  ": the block across it is not active at the same level");
       if (B.level > 1) {
         int fp[3] = {(p[0] + 1) / 2, (p[1] + 1) / 2, (p[2] + 1) / 2};
         auto itf = c->index.find((int)N.bid(B.level - 1, fp));
