@@ -50,6 +50,7 @@ TRAJ = {
     "amr_small": (8, [2, 8]),
     "snr_amr4": (5, [5]),
     "winds4": (5, [5]),
+    "ot64_amr": (8, [2, 8]),
 }
 NSAMPLE = 4096
 # workloads run with admesh after every step: (steps, dumped steps)

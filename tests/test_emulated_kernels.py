@@ -167,10 +167,10 @@ def test_emulated_admesh_reproduces_the_oracle(emu, tmp_path, oracle):
 
 
 def test_mhd_fused_on_mixed_levels(emu, tmp_path, oracle):
-    """The FP32 MHD stage kernels across coarse-fine faces (Orszag-Tang 64^3,
-    free boundaries, two families merged into level-2 blocks): every access
-    in bounds, and the state after 3 steps within the Orszag-Tang tolerances
-    of the oracle.  On the periodic domain the library refuses the mesh
+    """The FP32 MHD stage kernels across coarse-fine faces (Orszag-Tang on
+    two levels, free boundaries: the ot64_amr workload): every access in
+    bounds, and the state after 3 steps within the Orszag-Tang tolerances of
+    the oracle.  On the periodic domain the library refuses such a mesh
     (faces wrap onto blocks of another level)."""
     from conftest import mixed_level_ot, rel_l1
     w, b2, U2 = mixed_level_ot()
@@ -179,9 +179,9 @@ def test_mhd_fused_on_mixed_levels(emu, tmp_path, oracle):
     r = subprocess.run([emu, p, "1"], capture_output=True, text=True, timeout=300, cwd=str(tmp_path),
                        env=dict(os.environ, ASAN_OPTIONS="detect_stack_use_after_return=0:detect_leaks=0"))
     assert r.returncode != 0 and "not active at the same level" in r.stdout, r.stdout[-2000:]
-    w, b2, U2 = mixed_level_ot("free")
+    w = W.get("ot64_amr")                     # the same mesh on free boundaries
     p = str(tmp_path / "mixed.bin")
-    _write_input(p, w, hip.FUSED_FP32, state=(b2, U2))
+    b2, U2 = _write_input(p, w, hip.FUSED_FP32)
     _run(emu, [p, "3"], str(tmp_path))
     o = oracle.Sim(w, b2)
     o.set_state(U2)

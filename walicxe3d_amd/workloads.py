@@ -287,6 +287,14 @@ _reg(_w(name="ot64", mhd=True, solver="hlld", limiter="minmod", gamma_expr="5.0/
         gamma=5.0 / 3.0, cfl=0.4, visc_eta=0.0, eight_wave=True, bc=PER6,
         nroot=(1, 1, 1), maxlev=3, base_level=3, phys=(1.0, 1.0, 1.0), ic="ot", seed=3, nsteps=20,
         description="Orszag-Tang vortex 64^3 (reduced C3), periodic, HLLD"))
+# MHD across coarse-fine faces: Orszag-Tang on two levels (level-2 blocks 2
+# and 5 stay coarse, the other six are refined), free boundaries (a periodic
+# face must wrap onto a block of its own level)
+_reg(_w(name="ot64_amr", mhd=True, solver="hlld", limiter="minmod", gamma_expr="5.0/3.0",
+        gamma=5.0 / 3.0, cfl=0.4, visc_eta=0.0, eight_wave=True, bc=FREE6,
+        nroot=(1, 1, 1), maxlev=3, base_level=2, refine=(3, 4, 6, 7, 8, 9), phys=(1.0, 1.0, 1.0), ic="ot",
+        seed=3, nsteps=8,
+        description="Orszag-Tang vortex on 2 levels (2 blocks of 16^3 at level 2, 48 at level 3), free, HLLD"))
 # C3 full size (GPU only): 256^3 as 16^3 blocks of 16^3
 _reg(_w(name="ot256", mhd=True, solver="hlld", limiter="minmod", gamma_expr="5.0/3.0",
         gamma=5.0 / 3.0, cfl=0.4, visc_eta=0.0, eight_wave=True, bc=PER6,
