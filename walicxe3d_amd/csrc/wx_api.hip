@@ -10,7 +10,6 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
-#include <unordered_map>
 #include <vector>
 
 #include "wx_device.h"
@@ -54,6 +53,38 @@ struct DevBuf {
   }
 };
 
+// bID -> index with open addressing (linear probing); one allocation per
+// map, where std::unordered_map allocates per entry (the mesh's block
+// index and the admesh plan)
+struct BidMap {
+  std::vector<long> key;
+  std::vector<int> val;
+  size_t mask = 0;
+  BidMap() : BidMap(1) {}
+  explicit BidMap(size_t n) {
+    size_t cap = 16;
+    while (cap < 2 * n) cap <<= 1;
+    key.assign(cap, -1);
+    val.assign(cap, -1);
+    mask = cap - 1;
+  }
+  static size_t hash(long k) { return (size_t)(((uint64_t)k * 0x9E3779B97F4A7C15ull) >> 20); }
+  bool insert(long k, int v) {
+    size_t i = hash(k) & mask;
+    for (; key[i] != -1; i = (i + 1) & mask)
+      if (key[i] == k) return false;
+    key[i] = k;
+    val[i] = v;
+    return true;
+  }
+  int find(long k) const {
+    for (size_t i = hash(k) & mask; key[i] != -1; i = (i + 1) & mask)
+      if (key[i] == k) return val[i];
+    return -1;
+  }
+  bool count(long k) const { return find(k) >= 0; }
+};
+
 struct Ctx {
   wx_config cfg{};
   std::vector<double> tabT, tabC, winds;
@@ -65,7 +96,7 @@ struct Ctx {
   std::vector<double> dxyz;          // grid spacings per level (wx_set_mesh)
   std::vector<int> bids;
   std::vector<BlockInfo> info;
-  std::unordered_map<int, int> index;
+  BidMap index;                      // bID -> position in bids
   // device memory
   BlockInfo *d_info = nullptr;
   double *d_tabT = nullptr, *d_tabC = nullptr;
@@ -145,11 +176,11 @@ struct Numbering {
 int build_topology(Ctx *c, const int *bids) {
   Numbering N;
   N.init(c->nroot, c->maxlev);
-  c->index.clear();
+  c->index = BidMap(c->nblk);
   c->bids.assign(bids, bids + c->nblk);
   for (int b = 0; b < c->nblk; b++) {
     if (N.level(bids[b]) < 0) return fail(WX_ETOPO, "block ID outside the mesh: " + std::to_string(bids[b]));
-    if (!c->index.emplace(bids[b], b).second) return fail(WX_ETOPO, "duplicate block ID " + std::to_string(bids[b]));
+    if (!c->index.insert(bids[b], b)) return fail(WX_ETOPO, "duplicate block ID " + std::to_string(bids[b]));
   }
   c->info.assign(c->nblk, BlockInfo{});
   for (int b = 0; b < c->nblk; b++) {
@@ -174,10 +205,10 @@ int build_topology(Ctx *c, const int *bids) {
         wrapped = true;
       }
       long nid = N.bid(B.level, p);
-      auto it = c->index.find((int)nid);
-      if (it != c->index.end()) {
+      const int same = c->index.find(nid);
+      if (same >= 0) {
         F.type = NEIGH_SAME;
-        F.src[0] = it->second;
+        F.src[0] = same;
         continue;
       }
       // the reference fills no periodic ghosts (source/boundary.f90:594); the
@@ -189,10 +220,10 @@ int build_topology(Ctx *c, const int *bids) {
                                   ": the block across it is not active at the same level");
       if (B.level > 1) {
         int fp[3] = {(p[0] + 1) / 2, (p[1] + 1) / 2, (p[2] + 1) / 2};
-        auto itf = c->index.find((int)N.bid(B.level - 1, fp));
-        if (itf != c->index.end()) {
+        const int coarser = c->index.find(N.bid(B.level - 1, fp));
+        if (coarser >= 0) {
           F.type = NEIGH_COARSER;
-          F.src[0] = itf->second;
+          F.src[0] = coarser;
           continue;
         }
       }
@@ -206,11 +237,11 @@ int build_topology(Ctx *c, const int *bids) {
           ch[n] = 2 * p[n] - 1 + (low ? 1 : 0);
           ch[t1] = 2 * p[t1] - 1 + s1;
           ch[t2] = 2 * p[t2] - 1 + s2;
-          auto itc = c->index.find((int)N.bid(B.level + 1, ch));
-          if (itc == c->index.end())
+          const int finer = c->index.find(N.bid(B.level + 1, ch));
+          if (finer < 0)
             return fail(WX_ETOPO, "2:1 level balance violated at block " + std::to_string(B.bid) + " face " +
                                       std::to_string(f));
-          F.src[s1 + 2 * s2] = itc->second;
+          F.src[s1 + 2 * s2] = finer;
         }
     }
   }
@@ -284,9 +315,8 @@ int to_fortran_or_back(Ctx *c, const double *in, double *out, bool to_blocks, in
   // Fortran U(nbmax, nv, -1:18, -1:18, -1:18): slot fastest
   int nv = c->cfg.nv;
   for (int s = 0; s < nbmax; s++) {
-    auto it = c->index.find(lb[s]);
-    if (lb[s] == -1 || it == c->index.end()) continue;
-    int b = it->second;
+    const int b = lb[s] == -1 ? -1 : c->index.find(lb[s]);
+    if (b < 0) continue;
     for (int v = 0; v < nv; v++)
       for (int k = 1; k <= NC; k++)
         for (int j = 1; j <= NC; j++)
@@ -423,10 +453,10 @@ static int remap_table(Ctx *c, int nblk, const int *bids, std::vector<RemapSrc> 
     const long b = bids[n];
     const int lev = N.level(b);
     if (lev < 0) return fail(WX_ETOPO, "block ID outside the mesh: " + std::to_string(b));
-    auto it = c->index.find((int)b);
-    if (it != c->index.end()) {
+    const int same = c->index.find(b);
+    if (same >= 0) {
       m.kind = 0;
-      m.src = it->second;
+      m.src = same;
       continue;
     }
     int bc[3];
@@ -436,10 +466,10 @@ static int remap_table(Ctx *c, int nblk, const int *bids, std::vector<RemapSrc> 
     bool found = false;
     for (int l = lev - 1, d = 1; l >= 1; l--, d++) {
       for (int a = 0; a < 3; a++) p[a] = (p[a] + 1) / 2;
-      auto ia = c->index.find((int)N.bid(l, p));
-      if (ia != c->index.end()) {
+      const int anc = c->index.find(N.bid(l, p));
+      if (anc >= 0) {
         m.kind = 1;
-        m.src = ia->second;
+        m.src = anc;
         m.shift = d;
         for (int a = 0; a < 3; a++) {
           m.f0[a] = (bc[a] - 1) * NC;
@@ -454,10 +484,10 @@ static int remap_table(Ctx *c, int nblk, const int *bids, std::vector<RemapSrc> 
     m.kind = 2;
     for (int q = 0; q < 8; q++) {
       int ch[3] = {2 * bc[0] - 1 + (q & 1), 2 * bc[1] - 1 + ((q >> 1) & 1), 2 * bc[2] - 1 + ((q >> 2) & 1)};
-      auto ic = c->index.find((int)N.bid(lev + 1, ch));
-      if (ic == c->index.end())
+      const int kid = c->index.find(N.bid(lev + 1, ch));
+      if (kid < 0)
         return fail(WX_ETOPO, "block " + std::to_string(b) + " needs a restriction over more than one level");
-      m.kids[q] = ic->second;
+      m.kids[q] = kid;
     }
   }
   return WX_OK;
@@ -484,35 +514,6 @@ __global__ void __launch_bounds__(256) k_carry_prims(int nv, const RemapSrc *map
 // ---------------------------------------------------------------------------
 enum { FLAG_NONE = 0, FLAG_REFINE = 1, FLAG_COARSE = 2 };   // constants.f90:76-78
 
-// bID -> index with open addressing (linear probing); one allocation per
-// plan, where std::unordered_map allocates per entry
-struct BidMap {
-  std::vector<long> key;
-  std::vector<int> val;
-  size_t mask = 0;
-  explicit BidMap(size_t n) {
-    size_t cap = 16;
-    while (cap < 2 * n) cap <<= 1;
-    key.assign(cap, -1);
-    val.assign(cap, -1);
-    mask = cap - 1;
-  }
-  static size_t hash(long k) { return (size_t)(((uint64_t)k * 0x9E3779B97F4A7C15ull) >> 20); }
-  bool insert(long k, int v) {
-    size_t i = hash(k) & mask;
-    for (; key[i] != -1; i = (i + 1) & mask)
-      if (key[i] == k) return false;
-    key[i] = k;
-    val[i] = v;
-    return true;
-  }
-  int find(long k) const {
-    for (size_t i = hash(k) & mask; key[i] != -1; i = (i + 1) & mask)
-      if (key[i] == k) return val[i];
-    return -1;
-  }
-  bool count(long k) const { return find(k) >= 0; }
-};
 
 static long amr_father(const Numbering &N, long bid) {
   const int lev = N.level(bid);
@@ -651,8 +652,7 @@ static int amr_plan(const Numbering &N, const std::vector<int> &bids, std::vecto
 
 // face table of a block list without touching a context (2:1 balance and
 // the periodic partners checked as wx_set_mesh does)
-static int check_mesh(const Ctx *c, const std::vector<int> &bids) {
-  Ctx t;
+static int check_mesh(const Ctx *c, const std::vector<int> &bids, Ctx &t) {
   t.cfg = c->cfg;
   t.maxlev = c->maxlev;
   for (int d = 0; d < 3; d++) t.nroot[d] = c->nroot[d];
@@ -755,17 +755,25 @@ int wx_finalize(void *ctx) {
   return WX_OK;
 }
 
-int wx_set_mesh(void *ctx, const int nroot[3], int maxlev, const double *dxyz, int nblk, const int *bids) {
-  Ctx *c = as_ctx(ctx);
-  if (!c || !nroot || !dxyz || !bids) return fail(WX_EINVAL, "null argument");
-  if (maxlev < 1 || maxlev > MAXLEV || nblk < 1) return fail(WX_EINVAL, "bad mesh size");
+// wx_set_mesh; `pre`, when given, holds the face table of this block list
+// already built (check_mesh with the context's nroot, maxlev and boundaries),
+// which is taken over instead of built again
+static int set_mesh_impl(Ctx *c, const int nroot[3], int maxlev, const double *dxyz, int nblk, const int *bids,
+                         Ctx *pre) {
   HIPCHK(hipStreamSynchronize(c->st));
   free_mesh(c);
   c->nblk = nblk;
   c->maxlev = maxlev;
   for (int d = 0; d < 3; d++) c->nroot[d] = nroot[d];
   if (dxyz != c->dxyz.data()) c->dxyz.assign(dxyz, dxyz + 3 * maxlev);
-  int rc = build_topology(c, bids);
+  int rc = WX_OK;
+  if (pre) {
+    c->bids = std::move(pre->bids);
+    c->info = std::move(pre->info);
+    c->index = std::move(pre->index);
+  } else {
+    rc = build_topology(c, bids);
+  }
   if (rc != WX_OK) return rc;
   size_t n = (size_t)nblk * c->cfg.nv * NCELL;
   HIPCHK(hipMalloc(&c->d_info, sizeof(BlockInfo) * nblk));
@@ -788,6 +796,13 @@ int wx_set_mesh(void *ctx, const int nroot[3], int maxlev, const double *dxyz, i
     if (frc != 0) return fail(WX_ENOMEM, "fused-form allocation failed");
   }
   return WX_OK;
+}
+
+int wx_set_mesh(void *ctx, const int nroot[3], int maxlev, const double *dxyz, int nblk, const int *bids) {
+  Ctx *c = as_ctx(ctx);
+  if (!c || !nroot || !dxyz || !bids) return fail(WX_EINVAL, "null argument");
+  if (maxlev < 1 || maxlev > MAXLEV || nblk < 1) return fail(WX_EINVAL, "bad mesh size");
+  return set_mesh_impl(c, nroot, maxlev, dxyz, nblk, bids, nullptr);
 }
 
 // Interior state [nblk][nv][16^3] already on the device into the context:
@@ -1059,14 +1074,16 @@ int wx_last_timing(void *ctx, double *s1, double *s2, double *tot) {
   return WX_OK;
 }
 
-int wx_remesh(void *ctx, int nblk, const int *bids) {
-  Ctx *c = as_ctx(ctx);
-  int rc = need_state(c);
-  if (rc != WX_OK) return rc;
-  if (!bids || nblk < 1) return fail(WX_EINVAL, "bad block list");
+// wx_remesh on a context with state; `pre`, when given, is the new list's
+// face table that check_mesh already built
+static int remesh_impl(Ctx *c, int nblk, const int *bids, Ctx *pre) {
   std::vector<RemapSrc> table;
-  rc = remap_table(c, nblk, bids, table);
-  if (rc == WX_OK) rc = check_mesh(c, std::vector<int>(bids, bids + nblk));
+  int rc = remap_table(c, nblk, bids, table);
+  Ctx checked;
+  if (rc == WX_OK && !pre) {
+    rc = check_mesh(c, std::vector<int>(bids, bids + nblk), checked);
+    pre = &checked;
+  }
   if (rc != WX_OK) return rc;              // the current mesh and state are untouched
   const int nv = c->cfg.nv;
   HIPCHK(hipStreamSynchronize(c->st));
@@ -1087,7 +1104,7 @@ int wx_remesh(void *ctx, int nblk, const int *bids) {
     if (rc == WX_OK) {
       const std::vector<double> dxyz = c->dxyz;
       int nroot[3] = {c->nroot[0], c->nroot[1], c->nroot[2]};
-      rc = wx_set_mesh(ctx, nroot, c->maxlev, dxyz.data(), nblk, bids);
+      rc = set_mesh_impl(c, nroot, c->maxlev, dxyz.data(), nblk, bids, pre);
     }
     if (rc != WX_OK) return rc;
     return upload_device(c, d_new.release(), true);   // the fused state owns it now
@@ -1115,7 +1132,7 @@ int wx_remesh(void *ctx, int nblk, const int *bids) {
   if (rc == WX_OK) {
     const std::vector<double> dxyz = c->dxyz;
     int nroot[3] = {c->nroot[0], c->nroot[1], c->nroot[2]};
-    rc = wx_set_mesh(ctx, nroot, c->maxlev, dxyz.data(), nblk, bids);
+    rc = set_mesh_impl(c, nroot, c->maxlev, dxyz.data(), nblk, bids, pre);
   }
   if (rc >= WX_OK) rc = upload_device(c, d_new.p);
   if (rc >= WX_OK && carry) {
@@ -1125,6 +1142,14 @@ int wx_remesh(void *ctx, int nblk, const int *bids) {
     if (rc2 != WX_OK) rc = rc2;
   }
   return rc;
+}
+
+int wx_remesh(void *ctx, int nblk, const int *bids) {
+  Ctx *c = as_ctx(ctx);
+  int rc = need_state(c);
+  if (rc != WX_OK) return rc;
+  if (!bids || nblk < 1) return fail(WX_EINVAL, "bad block list");
+  return remesh_impl(c, nblk, bids, nullptr);
 }
 
 int wx_admesh_plan(const int nroot[3], int maxlev, int nblk, const int *bids, const int *flags, int *flags_out,
@@ -1296,10 +1321,11 @@ int wx_admesh(void *ctx, double refine_thres, double coarse_thres, int cap, int 
     std::copy(out.begin(), out.end(), bids_new);
     return WX_OK;
   }
-  rc = check_mesh(c, out);
+  Ctx checked;
+  rc = check_mesh(c, out, checked);
   if (rc != WX_OK) return rc;
   std::copy(out.begin(), out.end(), bids_new);
-  return wx_remesh(ctx, (int)out.size(), out.data());
+  return remesh_impl(c, (int)out.size(), out.data(), &checked);
 }
 
 }  // extern "C"
