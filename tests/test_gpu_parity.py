@@ -306,7 +306,7 @@ def test_fused_remesh_recaptures_graph():
     h.close()
 
 
-@pytest.mark.parametrize("name", ["amr_small", "ot64"])
+@pytest.mark.parametrize("name", ["amr_small", "ot64", "ot64_amr"])
 def test_fused_launch_and_order_choices_leave_results_unchanged(name):
     """The autotuner's choices (persistent grid, XCD ranges, Morton or
     Hilbert block order) change only which workgroup updates which block
