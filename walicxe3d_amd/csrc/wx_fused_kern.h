@@ -289,7 +289,10 @@ __global__ void __launch_bounds__(256, (NV == 8 && !S2) ? WX_MHD_STAGE1_MIN_WAVE
   constexpr int ORD = S2 ? 2 : 1;
   constexpr int H = ORD;
   constexpr int W = NC + 2 * H;
-  constexpr int WP = W | 1;                    // odd LDS row pitch (bank spread of y-direction reads)
+  // LDS row pitch: odd (bank spread of y-direction reads); in the FP32 MHD
+  // second stage without viscosity 49, which is 17 mod 32, so a half-wave's
+  // x-face reads (rows of 17 interfaces) fall in distinct banks
+  constexpr int WP = (NV == 8 && S2 && !VISC && sizeof(RF) == 4) ? 49 : (W | 1);
   constexpr int NW = S2 ? 4 : 3;               // planes in the column window
   constexpr bool SV = S2 && VISC;
   constexpr int NXF = (NC + 1) * NC;           // x (or y) interfaces per plane
