@@ -15,6 +15,14 @@ coarsening threshold that merges part of the mesh, in the fused form, and
 wx_mark_blocks in the plain form; under `rocprofv3 --kernel-trace --stats`
 the per-kernel times of the same calls.
 
+Sensitivity mode:
+    python3 -m walicxe3d_amd.profile_run sensitivity --workload ot256 --steps 1000 --rel 1e-15
+runs the plain FP64 form twice from the workload's initial state, the second
+time with every conserved value scaled by (1 + rel), and prints the relative
+L1 difference of the two states per variable after `steps` steps (the
+measure bench.py applies to the fused FP32 form): how much the flow itself
+amplifies a rounding-size difference.
+
 Summary mode:
     python3 -m walicxe3d_amd.profile_run summarize --fetch DIR --write DIR \
         --workload ot256 --out profiles/stage2_ot256_traffic.txt [--cmd "..."]
@@ -193,6 +201,38 @@ def _summarize(args):
     return 0
 
 
+def _sensitivity(args):
+    import json
+
+    import numpy as np
+
+    from walicxe3d_amd import hip
+    from walicxe3d_amd import workloads as W
+
+    w = W.get(args.workload)
+    slots, blocks, U0 = W.initial_state(w)
+    bids = np.asarray(blocks, dtype=np.int32)
+    out = []
+    for scale in (1.0, 1.0 + args.rel):
+        g = hip.Walicxe3D(w, form=hip.PLAIN_FP64, bids=bids)
+        g.upload(U0 * scale)
+        for n in range(args.steps):
+            g.step()
+            if (n + 1) % 100 == 0:
+                print("  scale %.17g: step %d" % (scale, n + 1), flush=True)
+        out.append(g.download(prim=False))
+        g.close()
+    d = np.abs(out[1] - out[0]).sum(axis=(0, 2, 3, 4))
+    s = np.abs(out[0]).sum(axis=(0, 2, 3, 4))
+    rel = np.where(s > 0, d / np.maximum(s, 1e-300), d)
+    names = ["rho", "mx", "my", "mz", "E", "bx", "by", "bz"][: w.nv]
+    print("profile_run sensitivity " + json.dumps({
+        "workload": w.name, "steps": args.steps, "initial_rel_perturbation": args.rel,
+        "rel_l1_all_conserved": float("%.3e" % (d.sum() / s.sum())),
+        "rel_l1": {n: float("%.3e" % v) for n, v in zip(names, rel)}}), flush=True)
+    return 0
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="mode", required=True)
@@ -202,6 +242,10 @@ def main(argv=None):
     a = sub.add_parser("admesh")
     a.add_argument("--workload", default="winds32k")
     a.add_argument("--reps", type=int, default=3)
+    t = sub.add_parser("sensitivity")
+    t.add_argument("--workload", default="ot256")
+    t.add_argument("--steps", type=int, default=1000)
+    t.add_argument("--rel", type=float, default=1e-15)
     s = sub.add_parser("summarize")
     s.add_argument("--fetch", required=True)
     s.add_argument("--write", required=True)
@@ -211,6 +255,8 @@ def main(argv=None):
     args = ap.parse_args(argv)
     if args.mode == "admesh":
         return _admesh(args)
+    if args.mode == "sensitivity":
+        return _sensitivity(args)
     return _run(args) if args.mode == "run" else _summarize(args)
 
 
