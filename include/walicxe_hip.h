@@ -205,6 +205,37 @@ int wx_admesh(void *ctx, double refine_thres, double coarse_thres, int cap, int 
    load balance deals blocks to ranks in.  Host only. */
 int wx_hilbert_keys(const int nroot[3], int maxlev, int nblk, const int *bids, int *keys);
 
+/* The device implementations of single physics routines applied to n
+   records of host arrays (FP64, the plain form's arithmetic), for parity
+   tests against the reference's own per-routine outputs.  op:
+     WX_EVAL_FLOW2PRIM   flow2prim (source/hydro_core.f90:323-378):
+                         in nv -> out nv + 1 (primitives, then istat)
+     WX_EVAL_PRIM2FLOW   prim2flow (:278-313): in nv -> out nv
+     WX_EVAL_PRIM2FLUXES prim2fluxes (:387-442) along arg = 1, 2, 3 (x, y, z):
+                         in nv -> out nv
+     WX_EVAL_LIMITER     limiter (:613-701) with option arg: in 4 nv (pll, pl,
+                         pr, prr) -> out 2 nv (pl, pr)
+     WX_EVAL_RIEMANN     primfhll / primfhllc / primfhlle / primfhlld
+                         (source/hll.f90:175, hllc.f90:209, hlle.f90:166,
+                         hlld.f90:166) for solver arg: in 2 nv (pl, pr) ->
+                         out nv
+     WX_EVAL_RIEMANN_SEL primfhlld in the one-code-path form of the fused
+                         FP64 kernels: in 2 nv -> out nv
+     WX_EVAL_SPEEDS      sound, cfast (x, y, z), cfastX (:503-566): in nv ->
+                         out 5
+   cfg supplies nv, mhd, solver (primfhllc forms its contact speed only in
+   HLLC builds, source/hllc.f90:348-353), limiter, gamma and cv.  A
+   diagnostic entry point with no counterpart routine of its own in the
+   reference. */
+#define WX_EVAL_FLOW2PRIM 0
+#define WX_EVAL_PRIM2FLOW 1
+#define WX_EVAL_PRIM2FLUXES 2
+#define WX_EVAL_LIMITER 3
+#define WX_EVAL_RIEMANN 4
+#define WX_EVAL_RIEMANN_SEL 5
+#define WX_EVAL_SPEEDS 6
+int wx_eval_physics(const wx_config *cfg, int op, int arg, int n, const double *in, double *out);
+
 /* Checkpoint files of the reference, byte for byte.  writeBin
    (source/output.f90:156-214): the device state to a stream file (int32
    block count; per block int32 bID and U(1:neqtot,1:16,1:16,1:16), variable

@@ -58,4 +58,32 @@ def test_rejects_bad_config_before_touching_a_device():
     cfg.solver = 2   # first-order HLL runs only in the plain form
     assert L.wx_init(C.byref(cfg), C.byref(h)) == -1
     assert L.wx_set_mesh(None, None, 1, None, 1, None) == -1
+    # EOS_H_RATE reads the neutral fraction from a passive scalar, which the
+    # library does not carry
+    cfg = hip.make_config(W.get("amr_small"))
+    cfg.eos = 3
+    assert L.wx_init(C.byref(cfg), C.byref(h)) == -1
+    assert b"eos" in L.wx_last_error()
+    # the fused form runs HLLE/HLLD (and MHD) for 8 variables only
+    cfg = hip.make_config(W.get("ot64"), form=hip.FUSED_FP32)
+    cfg.solver = 4
+    assert L.wx_init(C.byref(cfg), C.byref(h)) == -1
+    cfg = hip.make_config(W.get("ot64"), form=hip.FUSED_FP32)
+    cfg.mhd = 0
+    assert L.wx_init(C.byref(cfg), C.byref(h)) == -1
+
+
+def test_download_checks_its_layout_arguments():
+    """wx_download_state refuses a Fortran layout without localBlocks or
+    nbMaxProc, and an unknown layout, before it touches any state."""
+    if not os.path.exists(hip.LIBPATH):
+        pytest.skip("library not built")
+    L = hip.load()
+    buf = (C.c_double * 8)()
+    assert L.wx_download_state(None, buf, None, hip.LAYOUT_FORTRAN, 0, None) == -1
+    assert b"localBlocks" in L.wx_last_error()
+    lb = (C.c_int * 1)(1)
+    assert L.wx_download_state(None, buf, None, hip.LAYOUT_FORTRAN, 0, lb) == -1
+    assert L.wx_download_state(None, buf, None, 7, 1, lb) == -1
+    assert b"layout" in L.wx_last_error()
     assert L.wx_step(None, None, None, None, None, 1.0, 1.0) == -1

@@ -704,6 +704,15 @@ int wx_init(const wx_config *cfg, void **out) {
     return fail(WX_EINVAL, "unknown form");
   if (cfg->form != WX_FORM_PLAIN_FP64 && cfg->solver == SOLVER_HLL1)
     return fail(WX_EINVAL, "the fused form runs the two-stage scheme (SOLVER_HLL1 needs the plain form)");
+  // the fused form instantiates HLL/HLLC for the 5 hydro variables and
+  // HLLE/HLLD for the 8 MHD variables only
+  if (cfg->form != WX_FORM_PLAIN_FP64 && cfg->nv == 8 && (cfg->solver != SOLVER_HLLE && cfg->solver != SOLVER_HLLD))
+    return fail(WX_EINVAL, "the fused form with nv = 8 runs HLLE or HLLD");
+  if (cfg->form != WX_FORM_PLAIN_FP64 && cfg->nv == 8 && !cfg->mhd)
+    return fail(WX_EINVAL, "the fused form with nv = 8 needs mhd = 1");
+  // calcTemp (source/hydro_core.f90:231-270): EOS_H_RATE (3) takes the
+  // neutral fraction from a passive scalar, which the library does not carry
+  if (cfg->eos < 0 || cfg->eos > 2) return fail(WX_EINVAL, "eos must be EOS_ADIABATIC, EOS_SINGLE_SPECIE or EOS_TWOTEMP");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(WX_EHIP, "no HIP device");
   Ctx *c = new Ctx();
@@ -882,6 +891,11 @@ int wx_upload_prims(void *ctx, const double *p, int layout, int nbmax, const int
 }
 
 int wx_download_state(void *ctx, double *u, double *prim, int layout, int nbmax, const int *lb) {
+  if (layout == WX_LAYOUT_FORTRAN) {
+    if (!lb || nbmax < 1) return fail(WX_EINVAL, "Fortran layout needs nbmax and localBlocks");
+  } else if (layout != WX_LAYOUT_BLOCKS) {
+    return fail(WX_EINVAL, "unknown layout");
+  }
   Ctx *c = as_ctx(ctx);
   int rc = need_state(c);
   if (rc != WX_OK) return rc;
@@ -1166,6 +1180,33 @@ int wx_admesh_plan(const int nroot[3], int maxlev, int nblk, const int *bids, co
   if ((int)out.size() > cap || !bids_new)
     return fail(WX_ENOMEM, "new block list (" + std::to_string(out.size()) + " blocks) exceeds the capacity");
   std::copy(out.begin(), out.end(), bids_new);
+  return WX_OK;
+}
+
+int wx_eval_physics(const wx_config *cfg, int op, int arg, int n, const double *in, double *out) {
+  if (!cfg || !in || !out || n < 0) return fail(WX_EINVAL, "null argument");
+  if (cfg->nv != 5 && cfg->nv != 8) return fail(WX_EINVAL, "nv must be 5 or 8");
+  if (op < WX_EVAL_FLOW2PRIM || op > WX_EVAL_SPEEDS) return fail(WX_EINVAL, "unknown op");
+  if (op == WX_EVAL_PRIM2FLUXES && (arg < 1 || arg > 3)) return fail(WX_EINVAL, "direction must be 1, 2 or 3");
+  if (op == WX_EVAL_LIMITER && (arg < -1 || arg > 6)) return fail(WX_EINVAL, "unknown limiter");
+  if (op == WX_EVAL_RIEMANN && (arg < SOLVER_HLL1 || arg > SOLVER_HLLD || (arg >= SOLVER_HLLE && cfg->nv != 8)))
+    return fail(WX_EINVAL, "unsupported solver");
+  if (op == WX_EVAL_RIEMANN_SEL && cfg->nv != 8) return fail(WX_EINVAL, "HLLD needs nv = 8");
+  if (n == 0) return WX_OK;
+  const int nv = cfg->nv;
+  const int nin = op == WX_EVAL_LIMITER ? 4 * nv : (op == WX_EVAL_RIEMANN || op == WX_EVAL_RIEMANN_SEL ? 2 * nv : nv);
+  const int nout = op == WX_EVAL_FLOW2PRIM ? nv + 1 : (op == WX_EVAL_LIMITER ? 2 * nv : (op == WX_EVAL_SPEEDS ? 5 : nv));
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(WX_EHIP, "no HIP device");
+  Phys<double> P{cfg->nv, cfg->mhd, cfg->solver, cfg->limiter, cfg->gamma, cfg->cv};
+  DevBuf<double> din, dout;
+  HIPCHK(din.alloc((size_t)n * nin));
+  HIPCHK(dout.alloc((size_t)n * nout));
+  HIPCHK(hipMemcpy(din.p, in, (size_t)n * nin * sizeof(double), hipMemcpyHostToDevice));
+  plain_eval(P, op, arg, n, din.p, dout.p, nullptr);
+  int rc = launch_check("eval");
+  if (rc != WX_OK) return rc;
+  HIPCHK(hipMemcpy(out, dout.p, (size_t)n * nout * sizeof(double), hipMemcpyDeviceToHost));
   return WX_OK;
 }
 

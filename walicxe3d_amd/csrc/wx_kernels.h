@@ -23,5 +23,9 @@ void plain_mark(const DevCfg &c, const double *P, double *maxgrad, hipStream_t s
 void fused_ghost_prims(const DevCfg &c, const double *U, double *G, hipStream_t st);
 void fused_mark(const DevCfg &c, const double *U, const double *G, double *maxgrad, hipStream_t st);
 size_t plain_flux_elems(int nblk, int nv);
+// device physics routines over arrays of records (wx_eval_physics)
+enum { EVAL_FLOW2PRIM = 0, EVAL_PRIM2FLOW, EVAL_PRIM2FLUXES, EVAL_LIMITER, EVAL_RIEMANN, EVAL_RIEMANN_SEL,
+       EVAL_SPEEDS };
+void plain_eval(const Phys<double> &P, int op, int arg, int n, const double *in, double *out, hipStream_t st);
 
 }  // namespace wx

@@ -11,6 +11,79 @@ namespace wx {
 constexpr int NIF = 17 * NC * NC;   // interfaces per direction per block used by upwindStep
 
 // ---------------------------------------------------------------------------
+// The device physics routines applied to arrays of records (one thread per
+// record), for parity tests of single routines.  op (EVAL_*):
+//   flow2prim   in nv -> out nv + 1 (primitives, istat)
+//   prim2flow   in nv -> out nv
+//   prim2fluxes in nv -> out nv, arg = direction 1..3
+//   limiter     in 4 nv (pll, pl, pr, prr) -> out 2 nv (pl, pr), arg = limiter
+//   riemann     in 2 nv (pl, pr) -> out nv, arg = solver (HLLD in the
+//               reference's branch form)
+//   riemann_sel in 2 nv -> out nv: HLLD in its select form (fused FP64 form)
+//   speeds      in nv -> out 5 (sound, cfast x/y/z, cfastX)
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_eval(Phys<double> P, int op, int arg, int n, const double *in,
+                                              double *out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int nv = P.nv;
+  double a[4 * MAXV], f[2 * MAXV];
+  const int nin = op == EVAL_LIMITER ? 4 * nv : (op == EVAL_RIEMANN || op == EVAL_RIEMANN_SEL ? 2 * nv : nv);
+  for (int q = 0; q < nin; q++) a[q] = in[(size_t)t * nin + q];
+  int nout = nv;
+  switch (op) {
+    case EVAL_FLOW2PRIM:
+      f[nv] = (double)flow2prim(P, a, f);
+      nout = nv + 1;
+      break;
+    case EVAL_PRIM2FLOW:
+      prim2flow(P, a, f);
+      break;
+    case EVAL_PRIM2FLUXES:
+      swapdim(nv, arg - 1, a);
+      xflux(P, a, f);
+      swapdim(nv, arg - 1, f);
+      break;
+    case EVAL_LIMITER:
+      for (int q = 0; q < nv; q++) {
+        double pl = a[nv + q], pr = a[2 * nv + q];
+        limit1(a[q], pl, pr, a[3 * nv + q], arg);
+        f[q] = pl;
+        f[nv + q] = pr;
+      }
+      nout = 2 * nv;
+      break;
+    case EVAL_RIEMANN:
+      // P keeps the build's solver: primfhllc forms its contact speed only in
+      // HLLC builds (source/hllc.f90:348-353)
+      if (arg == SOLVER_HLLC) primfhllc(P, a, a + nv, f);
+      else if (arg == SOLVER_HLLE) primfhlle(P, a, a + nv, f);
+      else if (arg == SOLVER_HLLD) primfhlld(P, a, a + nv, f);
+      else primfhll(P, a, a + nv, f);
+      break;
+    case EVAL_RIEMANN_SEL:
+      primfhlld_sel<double>(P, a, a + nv, f);
+      break;
+    default:   // EVAL_SPEEDS
+      f[0] = sound(P, a);
+      f[1] = f[2] = f[3] = f[4] = 0.0;
+      if (nv == 8) {
+        f[1] = cfast1(P, a[4], a[0], a[5], a[6], a[7], a[5]);
+        f[2] = cfast1(P, a[4], a[0], a[5], a[6], a[7], a[6]);
+        f[3] = cfast1(P, a[4], a[0], a[5], a[6], a[7], a[7]);
+        f[4] = cfastX(P, a);
+      }
+      nout = 5;
+      break;
+  }
+  for (int q = 0; q < nout; q++) out[(size_t)t * nout + q] = f[q];
+}
+
+void plain_eval(const Phys<double> &P, int op, int arg, int n, const double *in, double *out, hipStream_t st) {
+  hipLaunchKernelGGL(k_eval, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, P, op, arg, n, in, out);
+}
+
+// ---------------------------------------------------------------------------
 // boundary (source/boundary.f90:46-64): face ghosts of depth `depth`.
 // grid (6*depth, nblk), block 256 = transverse 16x16.
 // ---------------------------------------------------------------------------

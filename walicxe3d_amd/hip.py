@@ -16,7 +16,13 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIBPATH = os.environ.get("WX_LIB") or os.path.join(HERE, "libwalicxe_hip.so")
+LIBPATH = os.path.join(HERE, "libwalicxe_hip.so")
+if os.environ.get("WX_LIB"):
+    # another build of the same C ABI (A/B measurements); said loudly, so that
+    # no result is taken for the in-tree library's by mistake
+    LIBPATH = os.environ["WX_LIB"]
+    import sys as _sys
+    _sys.stderr.write("walicxe3d_amd.hip: WX_LIB is set, loading %s instead of the in-tree library\n" % LIBPATH)
 
 PLAIN_FP64 = 0
 FUSED_FP32 = 1
@@ -76,7 +82,20 @@ SIGNATURES = {
     "wx_write_state": (C.c_int, [C.c_char_p, C.c_double, C.c_double, C.c_int, C.c_int, C.c_char_p]),
     "wx_read_state": (C.c_int, [C.c_char_p, C.c_double, _dp, _ip, _ip, C.c_char_p, C.c_int]),
     "wx_write_vtk": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int, C.c_int, C.c_int]),
+    "wx_eval_physics": (C.c_int, [C.POINTER(WxConfig), C.c_int, C.c_int, C.c_int, _dp, _dp]),
 }
+
+EVAL_FLOW2PRIM, EVAL_PRIM2FLOW, EVAL_PRIM2FLUXES, EVAL_LIMITER, EVAL_RIEMANN, EVAL_RIEMANN_SEL, EVAL_SPEEDS = range(7)
+
+
+def eval_physics(cfg, op, arg, records, nout):
+    """The device implementation of one physics routine on every record
+    (rows of `records`); returns an array [n][nout] (wx_eval_physics)."""
+    a = np.ascontiguousarray(records, dtype=np.float64).reshape(len(records), -1)
+    out = np.empty((len(a), nout))
+    _check(load().wx_eval_physics(C.byref(cfg), op, arg, len(a), a.ctypes.data_as(_dp), out.ctypes.data_as(_dp)),
+           "wx_eval_physics")
+    return out
 
 FLAG_NONE, FLAG_REFINE, FLAG_COARSE = 0, 1, 2
 
