@@ -62,7 +62,7 @@ def read_ints(path):
 
 
 def run(w, nsteps, dumps, workdir=None, unit_op=None, unit_in=None, timeout=3600, tstop=-1.0, variant="",
-        nprocs=1, amr=False, bindump=False, vtkdump=False):
+        nprocs=1, amr=False, bindump=False, vtkdump=False, env=None):
     """Run the harness; returns dict with topo, steps (it, dt, time, tout),
     and states {step: (U, PRIM)} with the passive scalar stripped.  With
     nprocs > 1 the multi-rank build runs under mpiexec and only the step log
@@ -73,7 +73,8 @@ def run(w, nsteps, dumps, workdir=None, unit_op=None, unit_in=None, timeout=3600
     own checkpoint files of each dumped step come back as
     files {step: (Blocks000.NNNN.bin bytes, State.NNNN.dat text)}; with
     vtkdump vtk {step: (Blocks000.NNNN.vtk bytes, Grid.NNNN.vtk bytes)} and
-    visit (master.visit text at the end)."""
+    visit (master.visit text at the end).  env: extra environment variables
+    of the harness process (e.g. WALICXE_GPU_FORM for the drop-in builds)."""
     if nprocs > 1:
         variant = variant + "_np%d" % nprocs
     if amr:
@@ -106,7 +107,8 @@ def run(w, nsteps, dumps, workdir=None, unit_op=None, unit_in=None, timeout=3600
                    (" amr=.true.\n" if amr else "") + (" bindump=.true.\n" if bindump else "")
                    + (" vtkdump=.true.\n" if vtkdump else "")))
     cmd = [exe] if nprocs == 1 else [MPIEXEC, "-n", str(nprocs), exe]
-    r = subprocess.run(cmd, cwd=wd, timeout=timeout, capture_output=True, text=True)
+    r = subprocess.run(cmd, cwd=wd, timeout=timeout, capture_output=True, text=True,
+                       env=dict(os.environ, **env) if env else None)
     if r.returncode != 0:
         raise RuntimeError("reference harness failed (%d): %s %s" % (r.returncode, r.stdout[-2000:], r.stderr[-2000:]))
     if nprocs > 1:

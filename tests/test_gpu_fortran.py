@@ -1,8 +1,12 @@
 """The Fortran drop-in on the GPU: the reference's own driver and modules
 (built from the Walicxe3D sources) with module `hydro_solver` replaced by
 walicxe3d_amd/fortran/hydro_solver_gpu.f90, which calls the HIP library
-through ISO_C_BINDING.  Its trajectories must match the reference's golden
-fixtures (bit-exact without cooling; <= 1e-12 relative with cooling)."""
+through ISO_C_BINDING.  In the plain form its trajectories must match the
+reference's golden fixtures (bit-exact without cooling; <= 1e-12 relative
+with cooling).  In the fused FP32 form (WALICXE_GPU_FORM=fused; the
+reference's updatePrims and cooling still run on the host after each
+hydroSolver) the state must match the FP64 oracle within the fused form's
+tolerances, and the step sequence (dt, time) within FP32 rounding."""
 import os
 import sys
 
@@ -42,3 +46,38 @@ def test_fortran_dropin_matches_reference(name):
     print("  block-sum dev", dev_sum, "sample dev", dev_s, flush=True)
     tol = 1e-12 if w.cooling else 0.0
     assert dev_sum <= tol and dev_s <= tol
+
+
+# (workload, steps, tolerance on every variable, on the MHD z-momentum and
+# B_z): the tolerances of the fused FP32 form against the oracle
+FUSED_DROPIN = [("sedov", 10, 1e-6, None), ("ot64", 20, 1e-6, 5e-3), ("amr_small", 8, 1e-6, None),
+                ("blast_reflect", 30, 1e-6, None)]
+
+
+@pytest.mark.parametrize("name,nsteps,tol,tol_small", FUSED_DROPIN)
+def test_fortran_dropin_fused_form(oracle, name, nsteps, tol, tol_small):
+    import refrun
+    from conftest import ic_in_order, rel_l1
+    w = W.get(name)
+    exe = refrun.harness_path(name + "_gpu")
+    if not os.path.exists(exe):
+        pytest.skip("drop-in build missing (python oracle/build_ref.py --gpu)")
+    print("case fortran drop-in fused", name, "steps", nsteps, flush=True)
+    r = refrun.run(w, nsteps, [nsteps], variant="_gpu", timeout=300, env={"WALICXE_GPU_FORM": "fused"})
+    bids = r["topo"]["bids"]
+    o = oracle.Sim(w, bids)
+    o.set_state(ic_in_order(w, bids))
+    ref_steps = []
+    for _ in range(nsteps):
+        o.step()
+        ref_steps.append((o.dt.value, o.time.value))
+    steps = r["steps"]
+    assert len(steps) == nsteps
+    np.testing.assert_allclose(steps[:, 1:3], np.array(ref_steps), rtol=1e-5, atol=0)
+    u, _ = r["states"][nsteps]
+    uo, _ = o.get_state()
+    e = rel_l1(u, uo)
+    print("  rel L1 per variable", [float("%.2e" % x) for x in e], flush=True)
+    small = [3, 7] if (w.nv == 8 and tol_small is not None) else []
+    for v in range(w.nv):
+        assert e[v] <= (tol_small if v in small else tol), (v, e)

@@ -7,19 +7,29 @@
 !   doViscousStep()        source/hydro_solver.f90:217-250
 ! A Walicxe3D build links this file (and walicxe_hip.f90) instead of
 ! source/hydro_solver.f90 plus libwalicxe_hip.so; the rest of the code is
-! unchanged.  The integrator then runs on the GPU (the plain FP64 form, whose
-! results match the CPU path bit for bit):
+! unchanged.  The integrator then runs on the GPU:
 !   - the GPU context is created on the first call from the `parameters`
 !     module and the cooling table in `globals`;
 !   - the block list is (re)sent whenever localBlocks changes (AMR);
-!   - each call uploads U and the interior PRIM left by updatePrims/cooling,
-!     runs getTimestep/Godunov/doViscousStep on the device and downloads U,
-!     so updatePrims, cooling, admesh and output keep working on the host
-!     arrays as before.
-! Supported: npassive = 0 on the device (passive scalars, if any, are left
-! untouched), SOLVER_HLL1/HLL/HLLC/HLLE/HLLD, COOL_NONE/COOL_TABLE, one MPI
-! rank per GPU.  userBoundary effects other than periodic faces (handled by
-! the library) must be registered through hydro_gpu_set_winds.
+!   - plain form (default; results match the CPU path bit for bit): each
+!     call uploads U and the interior PRIM left by updatePrims/cooling, runs
+!     getTimestep/Godunov/doViscousStep on the device and downloads U;
+!   - fused forms (hydro_gpu_set_form, or WALICXE_GPU_FORM=fused|fused_fp64
+!     in the environment): one call is one device step of the fused stage
+!     kernels (getTimestep with its ramp and output clamp, both stages and
+!     the viscosity), then U comes back.  The state stays on the device
+!     between calls: it is sent again only after the block list changed,
+!     after cooling (which the reference's main loop applies on the host
+!     right after hydroSolver) or after hydro_gpu_state_changed;
+! so updatePrims, cooling, admesh and output keep working on the host
+! arrays as before.
+! Supported: SOLVER_HLL1/HLL/HLLC/HLLE/HLLD (the fused forms: HLL/HLLC for
+! hydro, HLLE/HLLD for MHD), COOL_NONE/COOL_TABLE, one MPI rank per GPU.
+! Passive scalars are not carried on the device: a run whose passive scalars
+! are not all zero stops with a message (zero passives stay zero under the
+! reference's update, so they are left as they are).  userBoundary effects
+! other than periodic faces (handled by the library) must be registered
+! through hydro_gpu_set_winds.
 !===============================================================================
 module hydro_solver
 
@@ -29,8 +39,11 @@ module hydro_solver
 
   private
   public :: hydroSolver, getTimestep, doViscousStep, hydro_gpu_set_winds, hydro_gpu_release
+  public :: hydro_gpu_set_form, hydro_gpu_state_changed
 
   type(c_ptr), save :: ctx = c_null_ptr
+  integer(c_int), save :: gpu_form = -1      ! WX_FORM_*, fixed at the first call
+  logical, save :: dev_current = .false.    ! the device holds the state of the host arrays
   integer(c_int), allocatable, target, save :: mesh_bids(:)
   integer(c_int), allocatable, target, save :: lbuf(:)
   real(c_double), allocatable, target, save :: ubuf(:,:,:,:,:)
@@ -57,7 +70,64 @@ contains
     if (c_associated(ctx)) rc = wx_finalize(ctx)
     ctx = c_null_ptr
     if (allocated(mesh_bids)) deallocate(mesh_bids)
+    dev_current = .false.
   end subroutine hydro_gpu_release
+
+  !> Form of the device update (WX_FORM_PLAIN_FP64, WX_FORM_FUSED_FP32 or
+  !! WX_FORM_FUSED_FP64); call before the first hydroSolver.
+  subroutine hydro_gpu_set_form(form)
+    integer, intent(in) :: form
+    if (c_associated(ctx)) call check(-1_c_int, "hydro_gpu_set_form (after the first hydroSolver)")
+    if (form /= WX_FORM_PLAIN_FP64 .and. form /= WX_FORM_FUSED_FP32 .and. form /= WX_FORM_FUSED_FP64) &
+      call check(-1_c_int, "hydro_gpu_set_form (unknown form)")
+    gpu_form = form
+  end subroutine hydro_gpu_set_form
+
+  !> The host arrays U were changed by other code than cooling or admesh:
+  !! the fused forms send them to the device again at the next call.
+  subroutine hydro_gpu_state_changed()
+    dev_current = .false.
+  end subroutine hydro_gpu_state_changed
+
+  !> The form from WALICXE_GPU_FORM unless hydro_gpu_set_form chose one.
+  subroutine choose_form()
+    character(len=32) :: v
+    integer :: l, st
+    if (gpu_form >= 0) return
+    gpu_form = WX_FORM_PLAIN_FP64
+    call get_environment_variable("WALICXE_GPU_FORM", v, l, st)
+    if (st /= 0) return
+    select case (trim(v))
+    case ("", "plain", "plain_fp64")
+      gpu_form = WX_FORM_PLAIN_FP64
+    case ("fused", "fused_fp32")
+      gpu_form = WX_FORM_FUSED_FP32
+    case ("fused_fp64")
+      gpu_form = WX_FORM_FUSED_FP64
+    case default
+      call check(-1_c_int, "WALICXE_GPU_FORM (plain, fused or fused_fp64)")
+    end select
+  end subroutine choose_form
+
+  !> Passive scalars are not carried on the device.  All-zero passives stay
+  !! zero under the reference's update, so they are left untouched; any other
+  !! value stops the run.
+  subroutine check_passives()
+    use parameters
+    use globals
+    use clean_quit, only : clean_abort
+    integer :: nb
+    if (npassive <= 0) return
+    do nb = 1, nbMaxProc
+      if (localBlocks(nb) /= -1) then
+        if (any(U(nb, firstpas:neqtot, :, :, :) /= 0.0)) then
+          write(logu,'(a,i0,a)') "passive scalars are not advected on the GPU (non-zero passive in block ", &
+                                 localBlocks(nb), ")"
+          call clean_abort (ERROR_GENERIC)
+        end if
+      end if
+    end do
+  end subroutine check_passives
 
   subroutine check(rc, what)
     use clean_quit, only : clean_abort
@@ -116,7 +186,13 @@ contains
       end if
       cfg%nwind = nwind
       if (nwind > 0) cfg%winds = c_loc(wind_rec)
-      cfg%form = WX_FORM_PLAIN_FP64
+      cfg%form = gpu_form
+      if (gpu_form /= WX_FORM_PLAIN_FP64) then
+        ! the reference's main loop cools on the host after hydroSolver
+        ! (source/main.f90:91-100): the fused step leaves cooling to it
+        cfg%cooling = 0
+        cfg%ntab = 0
+      end if
       call check(wx_init(cfg, ctx), "wx_init")
       allocate(ubuf(nbMaxProc, neqhydro + neqmhd, nxmin:nxmax, nymin:nymax, nzmin:nzmax))
       allocate(lbuf(nbMaxProc))
@@ -139,6 +215,7 @@ contains
     end if
     allocate(mesh_bids(nact))
     mesh_bids(:) = bids(:)
+    dev_current = .false.
     allocate(dxyz(3*maxlev))
     do l = 1, maxlev
       dxyz(l) = dx(l)
@@ -165,6 +242,18 @@ contains
                "wx_upload_prims")
   end subroutine to_device
 
+  !> U of the host arrays to the device (fused forms: PRIM is derived there).
+  subroutine to_device_u()
+    use parameters
+    use globals
+    integer :: nvg
+    nvg = neqhydro + neqmhd
+    lbuf(:) = localBlocks(1:nbMaxProc)
+    ubuf(:,:,:,:,:) = U(:, 1:nvg, :, :, :)
+    call check(wx_upload_state(ctx, c_loc(ubuf), WX_LAYOUT_FORTRAN, int(nbMaxProc,c_int), c_loc(lbuf)), &
+               "wx_upload_state")
+  end subroutine to_device_u
+
   !> Device U (interior) back into the host array.
   subroutine to_host()
     use parameters
@@ -183,7 +272,8 @@ contains
     use globals
     use clean_quit, only : clean_abort
     real :: dt_new
-    integer(c_int) :: order
+    integer(c_int) :: order, itc, noc
+    real(c_double) :: dtc, tc
 
     if (solver_type == SOLVER_LAX) then
       write(logu,'(a)') "SOLVER_LAX does not run on the GPU"
@@ -195,7 +285,25 @@ contains
       write(logu,'(a)') "user_source_terms: get_user_source_terms does not run on the GPU"
       call clean_abort (ERROR_GENERIC)
     end if
+    call check_passives()
+    call choose_form()
     call ensure_context()
+    if (gpu_form /= WX_FORM_PLAIN_FP64) then
+      ! one device step: dt (CFL minimum, start-up ramp, output clamp), both
+      ! stages and the viscosity; time advances as doViscousStep does
+      if (.not. dev_current .or. cooling_type /= COOL_NONE) call to_device_u()
+      itc = it - 1
+      dtc = dt
+      tc = time
+      noc = nextout
+      call check(wx_run(ctx, 1_c_int, itc, dtc, tc, noc, real(dtout, c_double), real(t_sc, c_double)), "wx_run")
+      dt = dtc
+      time = tc
+      if (noc /= nextout) dumpout = .true.
+      call to_host()
+      dev_current = .true.
+      return
+    end if
     call to_device()
 
     call getTimestep (dt_new)
