@@ -94,6 +94,36 @@ def cpu_baseline(budget_s=20.0):
                 "sample": "C restatement (OpenMP) on Orszag-Tang 64^3, %d steps" % n}
 
 
+def dropin_rate(name="ot64", nsteps=20):
+    """The reference's own driver and modules with the Fortran drop-in of
+    module hydro_solver in its fused FP32 form (oracle/_ref/<name>_gpu, the
+    harness of the drop-in GPU test; WALICXE_GPU_FORM=fused): zone-updates/s
+    of whole main-loop iterations (hydroSolver on the device, U over PCIe each
+    step, updatePrims and cooling on the host), start-up subtracted.  None
+    when that build is absent."""
+    sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
+    try:
+        import refrun
+        from walicxe3d_amd import workloads as W
+        w = W.get(name)
+        if not os.path.exists(refrun.harness_path(name + "_gpu")):
+            return None
+        env = {"WALICXE_GPU_FORM": "fused"}
+        t0 = time.time()
+        refrun.run(w, 1, [], variant="_gpu", env=env, timeout=300)
+        t_one = time.time() - t0
+        t0 = time.time()
+        refrun.run(w, nsteps + 1, [], variant="_gpu", env=env, timeout=300)
+        t = time.time() - t0 - t_one
+        zones = len(w.blocks()) * 4096
+        return {"value": zones * nsteps / t, "unit": "zone-updates/s",
+                "sample": "Walicxe3D driver + Fortran drop-in (fused FP32 form) on %s, %d steps, U over PCIe "
+                          "every step, updatePrims/cooling on the host, start-up subtracted" % (name, nsteps)}
+    except Exception as e:  # the drop-in is reported, never required
+        log("drop-in timing skipped: %s" % e)
+        return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -104,6 +134,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="skip the launch autotuner (profiling runs)")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the comparison forms (fused FP64, plain-FP32 MHD fluxes, Fortran drop-in)")
     ap.add_argument("--remesh-every", type=int, default=0,
                     help="AMR workloads: change the block list every N steps (device remesh, graph recapture), "
                          "alternating with a 6%% wider refinement")
@@ -196,6 +228,31 @@ def main():
         plain = {"value": plain_zps, "ms_per_step": 1e3 * t_plain / args.steps, "dtype": "f64",
                  "speedup_fused_over_plain": fused_zps / plain_zps}
 
+    # ---- comparison forms on the same workload ------------------------------
+    def timed(form, tuning=()):
+        g = hip.Walicxe3D(w, form=form, bids=bids)
+        for kind, value in tuning:
+            g.set_tuning(kind, value)
+        g.upload(U0)
+        g.run(args.warmup)
+        t0 = time.perf_counter()
+        g.run(args.steps)
+        t = time.perf_counter() - t0
+        g.close()
+        return zones * args.steps / t, 1e3 * t / args.steps
+
+    fused64 = plain32 = None
+    if not args.no_extra:
+        v, ms = timed(hip.FUSED_FP64, [(0, best["grid"]), (1, best["xcd_map"]), (2, best["curve"])])
+        fused64 = {"value": v, "ms_per_step": ms, "dtype": "f64",
+                   "speedup_fused_fp32_over_fused_fp64": fused_zps / v}
+        log("fused FP64: %.3f ms/step, %.3e zone-updates/s" % (ms, v))
+        if w.nv == 8:
+            v, ms = timed(hip.FUSED_FP32, [(0, best["grid"]), (1, best["xcd_map"]), (2, best["curve"]), (3, 1)])
+            plain32 = {"value": v, "ms_per_step": ms,
+                       "form": "fused FP32 with every flux term in FP32 (tuning kind 3 = 1)"}
+            log("fused FP32, plain FP32 fluxes: %.3f ms/step, %.3e zone-updates/s" % (ms, v))
+
     # ---- L1 error of the fused form against FP64 after err-steps steps -----
     err = None
     if not args.no_plain:
@@ -218,6 +275,18 @@ def main():
                "rel_l1_all_conserved": float("%.3e" % (d.sum() / s.sum())),
                "rel_l1": {n: float("%.3e" % v) for n, v in zip(names, rel)}, "max": float("%.3e" % rel.max())}
         log("fused rel L1 vs FP64 after %d steps: %s" % (args.err_steps, err["rel_l1"]))
+        if plain32 is not None:
+            c = hip.Walicxe3D(w, form=hip.FUSED_FP32, bids=bids)
+            c.set_tuning(3, 1)
+            c.upload(U0)
+            c.run(args.err_steps)
+            uc = c.download(prim=False)
+            c.close()
+            dc = np.abs(uc - ub).sum(axis=(0, 2, 3, 4))
+            relc = np.where(s > 0, dc / np.maximum(s, 1e-300), dc)
+            plain32["rel_l1"] = {n: float("%.3e" % v) for n, v in zip(names, relc)}
+            plain32["rel_l1_max"] = float("%.3e" % relc.max())
+            log("plain-FP32-flux rel L1 after %d steps: %s" % (args.err_steps, plain32["rel_l1"]))
 
     # ---- roofline of the dominant kernel (stage 2) ---------------------------
     nv = w.nv
@@ -225,12 +294,15 @@ def main():
     b_stage1 = zones * nv * (8 + 4)          # read U (f64), write UP (f32)
     dom_ms, dom_bytes, dom = (s2, b_stage2, "stage2") if s2 >= s1 else (s1, b_stage1, "stage1")
     achieved = dom_bytes / (dom_ms * 1e-3)
-    roof = {"bound": "hbm", "kernel": "k_stage<%d,%s>" % (nv, "S2" if dom == "stage2" else "S1"),
+    kname = ("k_mhd<%s>" % ("S2" if dom == "stage2" else "S1")) if nv == 8 else \
+        ("k_stage<%d,%s>" % (nv, "S2" if dom == "stage2" else "S1"))
+    roof = {"bound": "hbm", "kernel": kname,
             "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": achieved / HBM_PEAK,
             "traffic": traffic_from_profiles(w.name), "algorithmic_bytes_per_launch": dom_bytes,
             "avg_launch_ms": dom_ms}
     step_bytes = zones * nv * 32
     cpu = None if args.no_cpu else cpu_baseline()
+    dropin = None if (args.no_extra or args.no_cpu) else dropin_rate()
     out = {
         "metric": METRIC,
         "value": fused_zps,
@@ -245,13 +317,18 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": "%s: %s" % (w.name, w.description), "blocks": nblk, "zones": zones,
-                   "form": "fused FP32 bulk, FP64 conserved accumulation", "solver": w.solver,
+                   "form": ("fused FP32: FP64 primitives, face states and physical fluxes, FP32 Riemann correction "
+                            "(delta form), FP64 accumulation" if nv == 8 else
+                            "fused FP32 bulk, FP64 conserved accumulation"), "solver": w.solver,
                    "limiter": w.limiter, "step_hbm_roofline_frac": (fused_zps * step_bytes / zones) / HBM_PEAK,
                    "tuning": {"grid": best["grid"], "xcd_map": best["xcd_map"],
                               "curve": "hilbert" if best["curve"] else "morton"}, "remesh": remesh},
         "roofline": roof,
         "cpu_baseline": cpu,
         "plain_fp64": plain,
+        "fused_fp64": fused64,
+        "fused_fp32_plain_flux": plain32,
+        "dropin_fortran": dropin,
         "l1_err_vs_fp64": err,
     }
     print(json.dumps(out))

@@ -268,7 +268,9 @@ int wx_write_vtk(void *ctx, const char *datadir, int noutput, int npassive, int 
    kernel grid (0 one workgroup per block, else a persistent grid of that
    many workgroups), kind 1 = block order split into one curve range per XCD
    (1) or not (0), kind 2 = block order along the Morton (0) or Hilbert (1)
-   curve. */
+   curve, kind 3 = the FP32 MHD kernels' fluxes in delta form (0: FP64
+   physical flux plus FP32 Riemann correction, the default) or entirely in
+   FP32 (1: faster, with FP32 rounding of every flux term). */
 int wx_set_tuning(void *ctx, int kind, int value);
 /* Advances `reps` more iterations launching each stage kernel on its own
    between HIP events on the library's stream (fused form; no reference

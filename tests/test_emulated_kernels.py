@@ -170,7 +170,8 @@ def test_mhd_fused_on_mixed_levels(emu, tmp_path, oracle):
     """The FP32 MHD stage kernels across coarse-fine faces (Orszag-Tang on
     two levels, free boundaries: the ot64_amr workload): every access in
     bounds, and the state after 3 steps within the Orszag-Tang tolerances of
-    the oracle.  On the periodic domain the library refuses such a mesh
+    the oracle (1e-5 on the near-zero z-momentum and B_z, 1e-7 on the
+    other variables).  On the periodic domain the library refuses such a mesh
     (faces wrap onto blocks of another level)."""
     from conftest import mixed_level_ot, rel_l1
     w, b2, U2 = mixed_level_ot()
@@ -192,4 +193,4 @@ def test_mhd_fused_on_mixed_levels(emu, tmp_path, oracle):
     e = rel_l1(ue, uo)
     assert np.all(np.isfinite(ue))
     for v in range(w.nv):
-        assert e[v] <= (5e-3 if v in (3, 7) else 1e-6), (v, e)
+        assert e[v] <= (1e-5 if v in (3, 7) else 1e-7), (v, e)

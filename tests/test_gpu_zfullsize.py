@@ -3,7 +3,8 @@ oracle is too slow, through properties that do not depend on size:
 
   C3  Orszag-Tang 256^3 (4096 blocks): the fused kernels in FP64 equal the
       plain FP64 form (itself bit-exact with the oracle at 64^3) bit for bit;
-      the fused FP32 form conserves mass, momentum, energy and field to FP64
+      the fused FP32 form is within 1e-5 relative L1 of it on every variable
+      after 10 steps, and conserves mass, momentum, energy and field to FP64
       round-off on the periodic grid (Powell source off).
   C4  SNR on 4 AMR levels, 27,560 blocks, cooling, HLLC;
   C5  three winds on 5 AMR levels, 32,278 blocks, cooling, HLL:
@@ -67,6 +68,32 @@ def test_ot256_fused_fp64_equals_plain():
     assert a.time.value == b.time.value
     b.close()
     assert np.array_equal(ua, ub)
+
+
+def test_ot256_fused_fp32_per_variable_error():
+    """The stated precision target on the benchmark configuration: after 10
+    steps of Orszag-Tang 256^3 every conserved variable of the fused FP32
+    form is within 1e-5 relative L1 of the plain FP64 form (bit-exact with
+    the oracle), the near-zero z-momentum and B_z included."""
+    from walicxe3d_amd import hip
+    w = W.get("ot256")
+    bids = np.asarray(w.blocks(), dtype=np.int32)
+    say("case ot256 fused-fp32 vs plain-fp64 per variable, 10 steps")
+    U0 = ic_in_order(w, bids)
+    f = _solver(w, hip.FUSED_FP32, bids)
+    f.upload(U0)
+    f.run(10)
+    uf = f.download(prim=False)
+    f.close()
+    p = _solver(w, hip.PLAIN_FP64, bids)
+    p.upload(U0)
+    for _ in range(10):
+        p.step()
+    up = p.download(prim=False)
+    p.close()
+    e = rel_l1(uf, up)
+    say("  rel L1 per variable", [float("%.2e" % x) for x in e])
+    assert np.all(e <= 1e-5), e
 
 
 def test_ot256_fused_fp32_conserves():

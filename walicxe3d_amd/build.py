@@ -35,7 +35,7 @@ UNITS = {
     "wx_fused.hip": ["-ffp-contract=off", "-fno-hip-fp32-correctly-rounded-divide-sqrt"],
     # FP32 MHD stage kernels: FMA contraction on (same code path for every
     # face, so shared faces still round identically)
-    "wx_fused_mhd.hip": ["-ffp-contract=fast", "-fno-hip-fp32-correctly-rounded-divide-sqrt"],
+    "wx_fused_mhd.hip": ["-ffp-contract=fast-honor-pragmas", "-fno-hip-fp32-correctly-rounded-divide-sqrt"],
 }
 
 

@@ -25,7 +25,12 @@ struct StepState {
 };
 
 struct FusedState {
-  double *U = nullptr;       // [blk][nv][4096] FP64 conserved state
+  double *U = nullptr;       // [blk][nv][4096] FP64 conserved state (the current one)
+  // FP32 MHD form: UP holds FP32 increments over U^n, so the second stage
+  // needs U^n of neighbour blocks after their own update: it writes U^{n+1}
+  // into this second buffer, and the two swap roles every step
+  double *U2 = nullptr;
+  int two_u = 0;
   void *UP = nullptr;        // [blk][nv][4096] half-step state (FP32, or FP64 in the fp64 variant)
   int fp64 = 0;              // flux arithmetic and UP storage in FP64
   int *order = nullptr;      // block processing order (device)
@@ -34,9 +39,13 @@ struct FusedState {
   int grid = 0;              // stage-kernel grid size (0 = one workgroup per block)
   int xcd_map = 1;           // XCD-aware block order
   int curve = 0;             // block order along the Morton (0) or Hilbert (1) curve
+  int mhd_plain32 = 0;       // FP32 MHD: plain FP32 fluxes (1) instead of the delta form (0)
   int nroot[3] = {1, 1, 1};  // root blocks per axis (Hilbert curve order)
-  hipGraph_t graph = nullptr;
+  hipGraph_t graph = nullptr;      // one step from the current U
   hipGraphExec_t exec = nullptr;
+  hipGraph_t graph2 = nullptr;     // with two U buffers: the step from the other one
+  hipGraphExec_t exec2 = nullptr;
+  const double *graph_u = nullptr, *graph2_u = nullptr;   // the U each graph reads
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   double t_s1 = 0, t_s2 = 0, t_tot = 0;
   int timing_runs = 0;

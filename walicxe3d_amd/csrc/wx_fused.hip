@@ -148,8 +148,10 @@ int fused_alloc(FusedState &f, const DevCfg &c, const std::vector<BlockInfo> &in
   fused_free(f);
   f.nblk = c.nblk;
   f.nv = c.nv;
+  f.two_u = !f.fp64 && c.nv == 8;
   size_t n = (size_t)c.nblk * c.nv * NINT;
   FCHK(hipMalloc(&f.U, n * sizeof(double)));
+  if (f.two_u) FCHK(hipMalloc(&f.U2, n * sizeof(double)));
   FCHK(hipMalloc(&f.UP, n * (f.fp64 ? sizeof(double) : sizeof(float))));
   FCHK(hipMalloc(&f.order, c.nblk * sizeof(int)));
   FCHK(hipMalloc(&f.step, sizeof(StepState)));
@@ -161,12 +163,20 @@ int fused_alloc(FusedState &f, const DevCfg &c, const std::vector<BlockInfo> &in
   return 0;
 }
 
-void fused_free(FusedState &f) {
+static void drop_graphs(FusedState &f) {
   if (f.exec) hipGraphExecDestroy(f.exec);
   if (f.graph) hipGraphDestroy(f.graph);
-  f.exec = nullptr;
-  f.graph = nullptr;
+  if (f.exec2) hipGraphExecDestroy(f.exec2);
+  if (f.graph2) hipGraphDestroy(f.graph2);
+  f.exec = f.exec2 = nullptr;
+  f.graph = f.graph2 = nullptr;
+  f.graph_u = f.graph2_u = nullptr;
+}
+
+void fused_free(FusedState &f) {
+  drop_graphs(f);
   hipFree(f.U);
+  hipFree(f.U2);
   hipFree(f.UP);
   hipFree(f.order);
   hipFree(f.step);
@@ -174,6 +184,7 @@ void fused_free(FusedState &f) {
     if (f.ev[e]) hipEventDestroy(f.ev[e]);
   for (int e = 0; e < 4; e++) f.ev[e] = nullptr;
   f.U = nullptr;
+  f.U2 = nullptr;
   f.UP = nullptr;
   f.order = nullptr;
   f.step = nullptr;
@@ -183,7 +194,7 @@ static FusedArgs make_args(FusedState &f, const DevCfg &c) {
   FusedArgs a;
   a.c = c;
   a.U = f.U;
-  a.Uout = f.U;
+  a.Uout = f.U2 ? f.U2 : f.U;
   a.UPin = f.UP;
   a.UPout = f.UP;
   a.order = f.order;
@@ -236,10 +247,11 @@ int fused_timestep(FusedState &f, const DevCfg &c, hipStream_t st, double *dt_ra
 }
 
 template <typename RF, typename ST>
-static void launch_stage_t(const FusedArgs &a, int grid, int nv, int solver, bool visc, int stage, hipStream_t st) {
+static void launch_stage_t(const FusedArgs &a, int grid, int nv, int solver, bool visc, int stage, int plain32,
+                           hipStream_t st) {
   if (nv == 8) {
     if constexpr (sizeof(RF) == 4) {
-      launch_stage_mhd32(a, grid, solver, visc, stage, st);
+      launch_stage_mhd32(a, grid, solver, visc, stage, plain32, st);
     } else {
       if (solver == SOLVER_HLLD) launch_stage_s<RF, ST, 8, SOLVER_HLLD>(a, grid, visc, stage, st);
       else launch_stage_s<RF, ST, 8, SOLVER_HLLE>(a, grid, visc, stage, st);
@@ -254,8 +266,8 @@ static void launch_stage(FusedState &f, const DevCfg &c, hipStream_t st, int sta
   FusedArgs a = make_args(f, c);
   int grid = f.grid > 0 ? std::min(f.grid, f.nblk) : f.nblk;
   bool visc = c.visc_eta != 0.0;
-  if (f.fp64) launch_stage_t<double, double>(a, grid, c.nv, c.pd.solver, visc, stage, st);
-  else launch_stage_t<float, float>(a, grid, c.nv, c.pd.solver, visc, stage, st);
+  if (f.fp64) launch_stage_t<double, double>(a, grid, c.nv, c.pd.solver, visc, stage, 0, st);
+  else launch_stage_t<float, float>(a, grid, c.nv, c.pd.solver, visc, stage, f.mhd_plain32, st);
 }
 
 static void enqueue_step(FusedState &f, const DevCfg &c, hipStream_t st) {
@@ -263,6 +275,42 @@ static void enqueue_step(FusedState &f, const DevCfg &c, hipStream_t st) {
   launch_stage(f, c, st, 1);
   launch_stage(f, c, st, 2);
   hipLaunchKernelGGL(k_step_post, dim3(1), dim3(1), 0, st, f.step);
+}
+
+// after a step of the two-buffer form the new state is in the other buffer
+static void swap_u(FusedState &f) {
+  if (f.U2) std::swap(f.U, f.U2);
+}
+
+static int capture_step(FusedState &f, const DevCfg &c, hipStream_t st, hipGraph_t *g, hipGraphExec_t *e) {
+  FCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  enqueue_step(f, c, st);
+  hipError_t err = hipStreamEndCapture(st, g);
+  FCHK(err);
+  FCHK(hipGraphInstantiate(e, *g, nullptr, nullptr, 0));
+  return 0;
+}
+
+// The step graphs: ex[0] steps from the current U, ex[1] from the other
+// buffer of the two-buffer form (the same graph otherwise).  Captured again
+// whenever a buffer moved since the capture.
+static int step_graphs(FusedState &f, const DevCfg &c, hipStream_t st, hipGraphExec_t ex[2]) {
+  const bool ok = f.exec && f.graph_u == f.U && (!f.U2 || (f.exec2 && f.graph2_u == f.U2));
+  if (!ok) {
+    drop_graphs(f);
+    if (capture_step(f, c, st, &f.graph, &f.exec)) return 1;
+    f.graph_u = f.U;
+    if (f.U2) {
+      swap_u(f);
+      const int rc = capture_step(f, c, st, &f.graph2, &f.exec2);
+      swap_u(f);
+      if (rc) return 1;
+      f.graph2_u = f.U2;
+    }
+  }
+  ex[0] = f.exec;
+  ex[1] = f.U2 ? f.exec2 : f.exec;
+  return 0;
 }
 
 int fused_run(FusedState &f, const DevCfg &c, hipStream_t st, int nsteps, int *it, double *dt, double *time,
@@ -278,15 +326,13 @@ int fused_run(FusedState &f, const DevCfg &c, hipStream_t st, int nsteps, int *i
   s.dtout = dtout;
   s.t_sc = t_sc;
   FCHK(hipMemcpyAsync(f.step, &s, sizeof(s), hipMemcpyHostToDevice, st));
-  if (!f.exec) {
-    FCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    enqueue_step(f, c, st);
-    hipError_t e = hipStreamEndCapture(st, &f.graph);
-    FCHK(e);
-    FCHK(hipGraphInstantiate(&f.exec, f.graph, nullptr, nullptr, 0));
-  }
+  hipGraphExec_t ex[2];
+  if (step_graphs(f, c, st, ex)) return 1;
   FCHK(hipEventRecord(f.ev[0], st));
-  for (int n = 0; n < nsteps; n++) FCHK(hipGraphLaunch(f.exec, st));
+  for (int n = 0; n < nsteps; n++) {
+    FCHK(hipGraphLaunch(ex[n & 1], st));
+    swap_u(f);
+  }
   FCHK(hipEventRecord(f.ev[1], st));
   FCHK(hipMemcpyAsync(&s, f.step, sizeof(s), hipMemcpyDeviceToHost, st));
   FCHK(hipStreamSynchronize(st));
@@ -312,6 +358,7 @@ int fused_time_stages(FusedState &f, const DevCfg &c, hipStream_t st, int reps) 
     launch_stage(f, c, st, 2);
     FCHK(hipEventRecord(f.ev[2], st));
     hipLaunchKernelGGL(k_step_post, dim3(1), dim3(1), 0, st, f.step);
+    swap_u(f);
     FCHK(hipEventSynchronize(f.ev[2]));
     float m1, m2;
     hipEventElapsedTime(&m1, f.ev[0], f.ev[1]);
@@ -329,14 +376,14 @@ int fused_time_stages(FusedState &f, const DevCfg &c, hipStream_t st, int reps) 
 //         grid of that many workgroups striding over the block list)
 // kind 1: XCD-aware block order (1) or one range for all XCDs (0)
 // kind 2: block order along the Morton (0) or the Hilbert (1) curve
+// kind 3: FP32 MHD fluxes in delta form (0) or plain FP32 (1)
 int fused_set_tuning(FusedState &f, int kind, int value) {
-  if (kind < 0 || kind > 2) return 1;
-  if (kind == 2 && (value < 0 || value > 1)) return 1;
-  if (f.exec) {
-    hipGraphExecDestroy(f.exec);
-    hipGraphDestroy(f.graph);
-    f.exec = nullptr;
-    f.graph = nullptr;
+  if (kind < 0 || kind > 3) return 1;
+  if ((kind == 2 || kind == 3) && (value < 0 || value > 1)) return 1;
+  drop_graphs(f);
+  if (kind == 3) {
+    f.mhd_plain32 = value;
+    return 0;
   }
   if (kind == 0) {
     if (value < 0) return 1;

@@ -99,13 +99,21 @@ template <int NV, typename RF> __device__ __forceinline__ void rot(int d, RF *v)
 // WIN = 0: interior cells keep the state before the wind is re-imposed (the
 // reference's stage 1 reuses the interior primitives of updatePrims, computed
 // before boundary() re-imposes the sources; source/godunov.f90:77-83).
-template <typename T, int NV, bool WIN>
+// DEL: W holds the half-step state as FP32 increments over U^n (a.U), and
+// the value of an element is U^n + increment, formed in FP64.
+template <typename T, int NV, bool WIN, bool DEL = false>
 __device__ __forceinline__ void load_cell(const FusedArgs &a, const T *W, int b, int i, int j, int k, double *u) {
   bool in = (unsigned)(i - 1) < NC && (unsigned)(j - 1) < NC && (unsigned)(k - 1) < NC;
   if (in) {
     const T *Wb = W + (size_t)b * NV * NINT + lin(i, j, k);
+    if constexpr (DEL) {
+      const double *Ub = a.U + (size_t)b * NV * NINT + lin(i, j, k);
 #pragma unroll
-    for (int v = 0; v < NV; v++) u[v] = (double)Wb[v * NINT];
+      for (int v = 0; v < NV; v++) u[v] = Ub[v * NINT] + (double)Wb[v * NINT];
+    } else {
+#pragma unroll
+      for (int v = 0; v < NV; v++) u[v] = (double)Wb[v * NINT];
+    }
   } else {
     int f, g, a1, a2;
     if (i < 1) { f = LEFT; g = 1 - i; a1 = j; a2 = k; }
@@ -118,7 +126,35 @@ __device__ __forceinline__ void load_cell(const FusedArgs &a, const T *W, int b,
     ghost_source(a.c.info, a.c.bc, b, f, g, a1, a2, s);
     const int ci = s.c[0], cj = s.c[1], ck = s.c[2];
     const T *S = W + (size_t)s.blk * NV * NINT;
-    if (s.kind == 1) {
+    if constexpr (DEL) {
+      const double *SU = a.U + (size_t)s.blk * NV * NINT;
+      const int q0 = lin(ci, cj, ck);
+      // value of element q of the source block
+      auto el = [&](int q) -> double { return SU[q] + (double)S[q]; };
+      if (s.kind == 1) {
+#pragma unroll
+        for (int v = 0; v < NV; v++) {
+          const int q = v * NINT + q0;
+          // 2x2x2 average in the reference's summation order (x, then y, then z fastest)
+          double acc = 0.0;
+          acc = acc + el(q) / 8.0;
+          acc = acc + el(q + NC * NC) / 8.0;
+          acc = acc + el(q + NC) / 8.0;
+          acc = acc + el(q + NC + NC * NC) / 8.0;
+          acc = acc + el(q + 1) / 8.0;
+          acc = acc + el(q + 1 + NC * NC) / 8.0;
+          acc = acc + el(q + 1 + NC) / 8.0;
+          acc = acc + el(q + 1 + NC + NC * NC) / 8.0;
+          u[v] = acc;
+        }
+      } else {
+#pragma unroll
+        for (int v = 0; v < NV; v++) {
+          double val = el(v * NINT + q0);
+          u[v] = (s.kind == 2 && v == s.flip) ? -val : val;
+        }
+      }
+    } else if (s.kind == 1) {
       const int q0 = lin(ci, cj, ck);
 #pragma unroll
       for (int v = 0; v < NV; v++) {
@@ -579,7 +615,8 @@ inline void launch_stage_s(const FusedArgs &a, int grid, bool visc, int stage, h
 }
 
 
-// FP32 MHD stage kernels (HLLD, HLLE), defined in wx_fused_mhd.hip
-void launch_stage_mhd32(const FusedArgs &a, int grid, int solver, bool visc, int stage, hipStream_t st);
+// FP32 MHD stage kernels (HLLD, HLLE), defined in wx_fused_mhd.hip: the
+// delta-form kernels, or with plain32 the plain FP32 flux kernels above
+void launch_stage_mhd32(const FusedArgs &a, int grid, int solver, bool visc, int stage, int plain32, hipStream_t st);
 
 }  // namespace wx

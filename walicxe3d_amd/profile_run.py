@@ -174,9 +174,11 @@ def _summarize(args):
              "write_scale = %.4f" % wscale]
     out = {}
     for k in fetch:
-        if "k_stage" not in k:
+        if "k_stage" not in k and "k_mhd" not in k:
             continue
-        st = "stage2" if "true" in k.split("<", 1)[1].split(",")[1] else "stage1"
+        # the second-stage flag: first template argument of k_mhd, second of k_stage
+        targs = k.split("<", 1)[1].split(",")
+        st = "stage2" if "true" in targs[0 if "k_mhd" in k else 1] else "stage1"
         f = sum(v for _, v in fetch[k]) / len(fetch[k])
         wr = sum(v for _, v in write.get(k, [(0, 0.0)])) / max(1, len(write.get(k, [])))
         hbm = 1024.0 * (f * fscale + wr * wscale)
