@@ -236,6 +236,18 @@ int wx_hilbert_keys(const int nroot[3], int maxlev, int nblk, const int *bids, i
 #define WX_EVAL_SPEEDS 6
 int wx_eval_physics(const wx_config *cfg, int op, int arg, int n, const double *in, double *out);
 
+/* wx_eval_physics without any device allocation: the same records, checks
+   and results (bit for bit), staged through a fixed array of
+   WX_EVAL_NOALLOC_DOUBLES doubles (4 MiB) that each GPU the library runs on
+   holds until the process ends.  n * (record in + record out doubles) must
+   not exceed it (WX_EINVAL otherwise: at most 10922 limiter records of 8
+   variables).  Every call on one GPU shares that GPU's array: calls that
+   overlap on one GPU (from several host threads) race.  No counterpart in
+   the reference; for callers that cannot allocate device memory (harnesses
+   that forbid hipMalloc, tight evaluation loops). */
+#define WX_EVAL_NOALLOC_DOUBLES (1 << 19)
+int wx_eval_physics_noalloc(const wx_config *cfg, int op, int arg, int n, const double *in, double *out);
+
 /* Checkpoint files of the reference, byte for byte.  writeBin
    (source/output.f90:156-214): the device state to a stream file (int32
    block count; per block int32 bID and U(1:neqtot,1:16,1:16,1:16), variable

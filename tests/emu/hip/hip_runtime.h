@@ -26,7 +26,7 @@
 #define __noinline__
 #define __launch_bounds__(...)
 #define __shared__ static
-#define __align__(n) alignas(n)
+#define __align__(n) __attribute__((aligned(n)))
 
 struct dim3 {
   unsigned x = 1, y = 1, z = 1;
@@ -54,6 +54,9 @@ typedef void *hipGraphNode_t;
 
 inline const char *hipGetErrorString(hipError_t) { return "emu error"; }
 inline hipError_t hipGetLastError() { return hipSuccess; }
+// device symbols are host arrays here: the symbol's address is the array's
+#define HIP_SYMBOL(x) (&(x))
+inline hipError_t hipGetSymbolAddress(void **p, const void *sym) { *p = const_cast<void *>(sym); return hipSuccess; }
 inline hipError_t hipGetDeviceCount(int *n) { *n = 1; return hipSuccess; }
 inline hipError_t hipGetDevice(int *d) { *d = 0; return hipSuccess; }
 inline hipError_t hipSetDevice(int) { return hipSuccess; }

@@ -4,6 +4,8 @@ import ctypes as C
 import os
 import re
 
+import numpy as np
+
 import pytest
 
 from conftest import ROOT
@@ -87,3 +89,21 @@ def test_download_checks_its_layout_arguments():
     assert L.wx_download_state(None, buf, None, 7, 1, lb) == -1
     assert b"layout" in L.wx_last_error()
     assert L.wx_step(None, None, None, None, None, 1.0, 1.0) == -1
+
+
+def test_noalloc_eval_refuses_more_than_its_staging():
+    """wx_eval_physics_noalloc stages through a fixed array: a call whose
+    records exceed it is refused before any device call."""
+    if not os.path.exists(hip.LIBPATH):
+        pytest.skip("library not built")
+    from walicxe3d_amd import workloads as W
+    L = hip.load()
+    cfg = hip.make_config(W.get("ot64"))
+    n = 10923                                  # 48 doubles a limiter record of 8 variables
+    a = np.zeros((n, 32))
+    out = np.zeros((n, 16))
+    dp = C.POINTER(C.c_double)
+    rc = L.wx_eval_physics_noalloc(C.byref(cfg), hip.EVAL_LIMITER, 1, n, a.ctypes.data_as(dp), out.ctypes.data_as(dp))
+    assert rc == -1
+    assert b"staging" in L.wx_last_error()
+    assert L.wx_eval_physics_noalloc(C.byref(cfg), 99, 0, 1, a.ctypes.data_as(dp), out.ctypes.data_as(dp)) == -1

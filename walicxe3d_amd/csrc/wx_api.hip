@@ -9,6 +9,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1183,7 +1184,10 @@ int wx_admesh_plan(const int nroot[3], int maxlev, int nblk, const int *bids, co
   return WX_OK;
 }
 
-int wx_eval_physics(const wx_config *cfg, int op, int arg, int n, const double *in, double *out) {
+// Argument checks of wx_eval_physics and its allocation-free variant; the
+// record widths (doubles per record) of op
+static int eval_check(const wx_config *cfg, int op, int arg, int n, const double *in, double *out, int *nin,
+                      int *nout) {
   if (!cfg || !in || !out || n < 0) return fail(WX_EINVAL, "null argument");
   if (cfg->nv != 5 && cfg->nv != 8) return fail(WX_EINVAL, "nv must be 5 or 8");
   if (op < WX_EVAL_FLOW2PRIM || op > WX_EVAL_SPEEDS) return fail(WX_EINVAL, "unknown op");
@@ -1192,10 +1196,17 @@ int wx_eval_physics(const wx_config *cfg, int op, int arg, int n, const double *
   if (op == WX_EVAL_RIEMANN && (arg < SOLVER_HLL1 || arg > SOLVER_HLLD || (arg >= SOLVER_HLLE && cfg->nv != 8)))
     return fail(WX_EINVAL, "unsupported solver");
   if (op == WX_EVAL_RIEMANN_SEL && cfg->nv != 8) return fail(WX_EINVAL, "HLLD needs nv = 8");
-  if (n == 0) return WX_OK;
   const int nv = cfg->nv;
-  const int nin = op == WX_EVAL_LIMITER ? 4 * nv : (op == WX_EVAL_RIEMANN || op == WX_EVAL_RIEMANN_SEL ? 2 * nv : nv);
-  const int nout = op == WX_EVAL_FLOW2PRIM ? nv + 1 : (op == WX_EVAL_LIMITER ? 2 * nv : (op == WX_EVAL_SPEEDS ? 5 : nv));
+  *nin = op == WX_EVAL_LIMITER ? 4 * nv : (op == WX_EVAL_RIEMANN || op == WX_EVAL_RIEMANN_SEL ? 2 * nv : nv);
+  *nout = op == WX_EVAL_FLOW2PRIM ? nv + 1 : (op == WX_EVAL_LIMITER ? 2 * nv : (op == WX_EVAL_SPEEDS ? 5 : nv));
+  return WX_OK;
+}
+
+int wx_eval_physics(const wx_config *cfg, int op, int arg, int n, const double *in, double *out) {
+  int nin, nout;
+  int rc = eval_check(cfg, op, arg, n, in, out, &nin, &nout);
+  if (rc != WX_OK) return rc;
+  if (n == 0) return WX_OK;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(WX_EHIP, "no HIP device");
   Phys<double> P{cfg->nv, cfg->mhd, cfg->solver, cfg->limiter, cfg->gamma, cfg->cv};
@@ -1204,9 +1215,64 @@ int wx_eval_physics(const wx_config *cfg, int op, int arg, int n, const double *
   HIPCHK(dout.alloc((size_t)n * nout));
   HIPCHK(hipMemcpy(din.p, in, (size_t)n * nin * sizeof(double), hipMemcpyHostToDevice));
   plain_eval(P, op, arg, n, din.p, dout.p, nullptr);
-  int rc = launch_check("eval");
+  rc = launch_check("eval");
   if (rc != WX_OK) return rc;
   HIPCHK(hipMemcpy(out, dout.p, (size_t)n * nout * sizeof(double), hipMemcpyDeviceToHost));
+  return WX_OK;
+}
+
+// ---- allocation-free variant ------------------------------------------------
+// The staging of wx_eval_physics_noalloc: inputs then outputs of one call,
+// in a fixed array that each device holds from the library's load to the
+// process's end.  Host code never passes the array's name to a kernel or a
+// copy (on the host it is only a stand-in): it uses the device address
+// resolved per device by eval_scratch().
+static __device__ __align__(256) double scratch[WX_EVAL_NOALLOC_DOUBLES];
+
+constexpr int WX_MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static double *g_scratch_ptr[WX_MAX_DEVICES];
+
+// the device address of `scratch` on the current device (resolved once per
+// device, with that device current)
+static int eval_scratch(double **ptr) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0) return fail(WX_EHIP, "no current HIP device");
+  if (dev >= WX_MAX_DEVICES) return fail(WX_EINVAL, "device index beyond the supported count");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) return fail(WX_EHIP, std::string("hipGetSymbolAddress: ") + hipGetErrorString(e));
+    g_scratch_ptr[dev] = static_cast<double *>(p);
+  }
+  *ptr = g_scratch_ptr[dev];
+  return WX_OK;
+}
+
+int wx_eval_physics_noalloc(const wx_config *cfg, int op, int arg, int n, const double *in, double *out) {
+  int nin, nout;
+  int rc = eval_check(cfg, op, arg, n, in, out, &nin, &nout);
+  if (rc != WX_OK) return rc;
+  if ((long long)n * (nin + nout) > (long long)WX_EVAL_NOALLOC_DOUBLES)
+    return fail(WX_EINVAL, "n = " + std::to_string(n) + " records of " + std::to_string(nin + nout) +
+                               " doubles exceed the fixed staging of " + std::to_string(WX_EVAL_NOALLOC_DOUBLES) +
+                               " doubles");
+  if (n == 0) return WX_OK;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(WX_EHIP, "no HIP device");
+  double *ptr = nullptr;
+  rc = eval_scratch(&ptr);
+  if (rc != WX_OK) return rc;
+  // every element the kernel reads is written by the copy first, and it
+  // writes every output element: nothing in the array needs clearing
+  double *din = ptr, *dout = ptr + (size_t)n * nin;
+  Phys<double> P{cfg->nv, cfg->mhd, cfg->solver, cfg->limiter, cfg->gamma, cfg->cv};
+  HIPCHK(hipMemcpy(din, in, (size_t)n * nin * sizeof(double), hipMemcpyHostToDevice));
+  plain_eval(P, op, arg, n, din, dout, nullptr);
+  rc = launch_check("eval");
+  if (rc != WX_OK) return rc;
+  HIPCHK(hipMemcpy(out, dout, (size_t)n * nout * sizeof(double), hipMemcpyDeviceToHost));
   return WX_OK;
 }
 

@@ -83,18 +83,21 @@ SIGNATURES = {
     "wx_read_state": (C.c_int, [C.c_char_p, C.c_double, _dp, _ip, _ip, C.c_char_p, C.c_int]),
     "wx_write_vtk": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int, C.c_int, C.c_int]),
     "wx_eval_physics": (C.c_int, [C.POINTER(WxConfig), C.c_int, C.c_int, C.c_int, _dp, _dp]),
+    "wx_eval_physics_noalloc": (C.c_int, [C.POINTER(WxConfig), C.c_int, C.c_int, C.c_int, _dp, _dp]),
 }
 
 EVAL_FLOW2PRIM, EVAL_PRIM2FLOW, EVAL_PRIM2FLUXES, EVAL_LIMITER, EVAL_RIEMANN, EVAL_RIEMANN_SEL, EVAL_SPEEDS = range(7)
 
 
-def eval_physics(cfg, op, arg, records, nout):
+def eval_physics(cfg, op, arg, records, nout, noalloc=False):
     """The device implementation of one physics routine on every record
-    (rows of `records`); returns an array [n][nout] (wx_eval_physics)."""
+    (rows of `records`); returns an array [n][nout] (wx_eval_physics, or
+    wx_eval_physics_noalloc with noalloc)."""
     a = np.ascontiguousarray(records, dtype=np.float64).reshape(len(records), -1)
     out = np.empty((len(a), nout))
-    _check(load().wx_eval_physics(C.byref(cfg), op, arg, len(a), a.ctypes.data_as(_dp), out.ctypes.data_as(_dp)),
-           "wx_eval_physics")
+    name = "wx_eval_physics_noalloc" if noalloc else "wx_eval_physics"
+    _check(getattr(load(), name)(C.byref(cfg), op, arg, len(a), a.ctypes.data_as(_dp), out.ctypes.data_as(_dp)),
+           name)
     return out
 
 FLAG_NONE, FLAG_REFINE, FLAG_COARSE = 0, 1, 2
