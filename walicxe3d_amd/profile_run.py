@@ -30,6 +30,17 @@ reads the two counter_collection.csv files (FETCH_SIZE and WRITE_SIZE, KiB
 per dispatch), takes the copy kernel's readings as the scale (bytes moved /
 counter), and writes per-launch HBM traffic of the stage kernels next to
 their algorithmic bytes.
+
+Counter table mode:
+    python3 -m walicxe3d_amd.profile_run counters --dirs DIR [DIR ...] \
+        --workload ot256 --out profiles/valu_mix_ot256.txt [--cmd "..."] [--rc "..."]
+the mean of every counter per dispatch of each stage kernel, over the
+counter_collection.csv files under the given directories, and the same per
+zone (x 64 lanes / zones: the instructions each lane issues per cell).
+
+Kernel-trace mode:
+    python3 -m walicxe3d_amd.profile_run stats --csv FILE --out profiles/... [--cmd "..."] [--rc "..."]
+formats rocprofv3's --stats kernel summary (durations in milliseconds).
 """
 from __future__ import annotations
 
@@ -168,17 +179,17 @@ def _summarize(args):
     lines = ["# HBM traffic of the fused stage kernels, %s (%d blocks, %d zones, nv=%d)" %
              (w.name, len(w.blocks()), zones, w.nv),
              "# command: %s" % (args.cmd or "(see walicxe3d_amd/profile_run.py)"),
+             "# rocprofv3 exit status: %s" % (args.rc or "not recorded"),
              "# counters: FETCH_SIZE and WRITE_SIZE (KiB per dispatch), one rocprofv3 --pmc run each",
              "# scale: the same counters on a %d-byte device copy (copyBuffer): bytes per counter byte" % COPY_BYTES,
              "fetch_scale = %.4f" % fscale,
              "write_scale = %.4f" % wscale]
     out = {}
     for k in fetch:
-        if "k_stage" not in k and "k_mhd" not in k:
-            continue
         # the second-stage flag: first template argument of k_mhd, second of k_stage
-        targs = k.split("<", 1)[1].split(",")
-        st = "stage2" if "true" in targs[0 if "k_mhd" in k else 1] else "stage1"
+        st = _stage_of(k)
+        if st is None:
+            continue
         f = sum(v for _, v in fetch[k]) / len(fetch[k])
         wr = sum(v for _, v in write.get(k, [(0, 0.0)])) / max(1, len(write.get(k, [])))
         hbm = 1024.0 * (f * fscale + wr * wscale)
@@ -195,6 +206,71 @@ def _summarize(args):
                   "%s_traffic_over_algorithmic = %.3f" % (st, hbm / alg[st])]
     if "stage2" in out:
         lines += ["", "hbm_bytes_per_launch = %.4e   (stage 2, the dominant kernel)" % out["stage2"][3]]
+    text = "\n".join(lines) + "\n"
+    if args.out:
+        with open(args.out, "w") as f:
+            f.write(text)
+    sys.stdout.write(text)
+    return 0
+
+
+def _header(args, title):
+    lines = ["# " + title]
+    if args.cmd:
+        lines.append("# command: " + args.cmd)
+    if args.rc:
+        lines.append("# rocprofv3 exit status: " + args.rc)
+    return lines
+
+
+def _stage_of(k):
+    """'stage1' / 'stage2' for a fused stage kernel name, else None."""
+    if "k_stage" not in k and "k_mhd" not in k:
+        return None
+    targs = k.split("<", 1)[1].split(",")
+    return "stage2" if "true" in targs[0 if "k_mhd" in k else 1] else "stage1"
+
+
+def _counters(args):
+    from walicxe3d_amd import workloads as W
+
+    w = W.get(args.workload)
+    zones = len(w.blocks()) * 4096
+    rows = []
+    for d in args.dirs:
+        rows += _read(d)
+    per = collections.defaultdict(lambda: collections.defaultdict(list))
+    for r in rows:
+        st = _stage_of(r["Kernel_Name"])
+        if st:
+            per[(st, r["Kernel_Name"])][r["Counter_Name"]].append(float(r["Counter_Value"]))
+    lines = _header(args, "Counters of the fused stage kernels, %s (%d blocks, %d zones, nv=%d), mean per dispatch"
+                    % (w.name, len(w.blocks()), zones, w.nv))
+    lines.append("# per zone = counter x 64 lanes / zones (instructions each lane of a wave issues per cell)")
+    for (st, k) in sorted(per):
+        lines += ["", "[%s] %s" % (st, k)]
+        for cn in sorted(per[(st, k)]):
+            v = per[(st, k)][cn]
+            m = sum(v) / len(v)
+            lines.append("  %-28s %12.4e   per zone %9.1f" % (cn, m, m * 64.0 / zones))
+    text = "\n".join(lines) + "\n"
+    if args.out:
+        with open(args.out, "w") as f:
+            f.write(text)
+    sys.stdout.write(text)
+    return 0
+
+
+def _stats(args):
+    rows = list(csv.DictReader(open(args.csv)))
+    lines = _header(args, "rocprofv3 kernel summary (%s); durations in milliseconds" % os.path.basename(args.csv))
+    lines.append("%-80s %6s %10s %9s %9s %9s %6s" % ("kernel", "calls", "total_ms", "avg_ms", "min_ms", "max_ms",
+                                                   "pct"))
+    for r in rows:
+        name = r["Name"][:80]
+        lines.append("%-80s %6s %10.2f %9.3f %9.3f %9.3f %6.2f" % (
+            name, r["Calls"], float(r["TotalDurationNs"]) / 1e6, float(r["AverageNs"]) / 1e6,
+            float(r["MinNs"]) / 1e6, float(r["MaxNs"]) / 1e6, float(r["Percentage"])))
     text = "\n".join(lines) + "\n"
     if args.out:
         with open(args.out, "w") as f:
@@ -254,7 +330,23 @@ def main(argv=None):
     s.add_argument("--workload", default="ot256")
     s.add_argument("--out")
     s.add_argument("--cmd")
+    s.add_argument("--rc")
+    c = sub.add_parser("counters")
+    c.add_argument("--dirs", nargs="+", required=True)
+    c.add_argument("--workload", default="ot256")
+    c.add_argument("--out")
+    c.add_argument("--cmd")
+    c.add_argument("--rc")
+    k = sub.add_parser("stats")
+    k.add_argument("--csv", required=True)
+    k.add_argument("--out")
+    k.add_argument("--cmd")
+    k.add_argument("--rc")
     args = ap.parse_args(argv)
+    if args.mode == "counters":
+        return _counters(args)
+    if args.mode == "stats":
+        return _stats(args)
     if args.mode == "admesh":
         return _admesh(args)
     if args.mode == "sensitivity":
