@@ -1,8 +1,12 @@
 // Test infrastructure: CPU execution of the library's kernels.  All lanes of
 // a workgroup run as fibers on one OS thread (an x86-64 stack switch written
 // below); __syncthreads switches back to the scheduler, which resumes the
-// next lane, so every lane reaches each barrier before any passes it.  A
-// barrier that some lanes skip aborts with a message.
+// next lane, so every lane reaches each barrier before any passes it.
+// Checks: a barrier that some lanes skip, or lanes waiting at different
+// barriers, abort with a message; EMU_ORDER_SEED runs the lanes of each
+// barrier interval in a seeded random order (a result that changes with it
+// reads shared data another lane writes in the same interval); the library's
+// WX_EMU_CHECK index ranges abort with the failing lane.
 #include "hip/hip_runtime.h"
 #if defined(__has_feature)
 #if __has_feature(address_sanitizer)
@@ -52,13 +56,28 @@ static int g_cur = -1;
 static std::function<void()> *g_body = nullptr;
 static const void *g_main_bottom; static size_t g_main_size;
 
+namespace wx {
+void wx_emu_check_failed(const char *what, const char *file, int line) {
+  fprintf(stderr, "EMU: index check failed: %s at %s:%d (block %u %u %u, thread %u %u %u)\n", what, file, line,
+          blockIdx.x, blockIdx.y, blockIdx.z, threadIdx.x, threadIdx.y, threadIdx.z);
+  abort();
+}
+}  // namespace wx
+
 static void to_sched() {
   Fiber &F = g_fib[g_cur];
   __sanitizer_start_switch_fiber(&F.fake, g_main_bottom, g_main_size);
   emu_switch(&F.sp, g_sched_sp);
   __sanitizer_finish_switch_fiber(F.fake, nullptr, nullptr);
 }
-void __syncthreads() { to_sched(); }
+// call site of each lane's current barrier: every lane of a round must wait
+// at the same __syncthreads (a barrier that lanes reach from different
+// places of the kernel is divergence the GPU does not tolerate)
+static std::vector<void *> g_site;
+__attribute__((noinline)) void __syncthreads() {
+  g_site[g_cur] = __builtin_return_address(0);
+  to_sched();
+}
 double emu_shfl_xor(double v, int off) {
   g_xbuf[threadIdx.x] = v;
   __syncthreads();
@@ -84,6 +103,14 @@ void emu_run_grid(dim3 grid, dim3 block, std::function<void()> body) {
   blockDim = block;
   int nt = block.x * block.y * block.z;
   g_fib.resize(nt);
+  g_site.assign(nt, nullptr);
+  // EMU_ORDER_SEED: run the lanes of each barrier interval in a seeded
+  // random order instead of lane order (a kernel whose result depends on
+  // that order reads shared data another lane writes in the same interval)
+  static const char *seed_env = getenv("EMU_ORDER_SEED");
+  const unsigned long long seed = seed_env ? strtoull(seed_env, nullptr, 10) : 0;
+  std::vector<int> order(nt);
+  unsigned long long rng = seed * 0x9E3779B97F4A7C15ull + 1;
   const size_t SZ = 1 << 20;
   for (auto &f : g_fib) if (f.stack.size() < SZ) f.stack.resize(SZ);
   g_body = &body;
@@ -104,7 +131,15 @@ void emu_run_grid(dim3 grid, dim3 block, std::function<void()> body) {
         int alive = nt;
         while (alive > 0) {
           int ndone = 0, nbar = 0;
-          for (int t = 0; t < nt; t++) {
+          for (int t = 0; t < nt; t++) order[t] = t;
+          if (seed_env) {
+            for (int t = nt - 1; t > 0; t--) {
+              rng ^= rng << 13; rng ^= rng >> 7; rng ^= rng << 17;
+              std::swap(order[t], order[rng % (unsigned long long)(t + 1)]);
+            }
+          }
+          for (int n = 0; n < nt; n++) {
+            const int t = order[n];
             if (g_fib[t].done) { ndone++; continue; }
             g_cur = t;
             threadIdx = dim3(t % block.x, (t / block.x) % block.y, t / (block.x * block.y));
@@ -115,6 +150,17 @@ void emu_run_grid(dim3 grid, dim3 block, std::function<void()> body) {
             if (g_fib[t].done) ndone++; else nbar++;
           }
           if (ndone > 0 && nbar > 0) { fprintf(stderr, "EMU: divergent barrier (%d done, %d waiting)\n", ndone, nbar); abort(); }
+          if (nbar > 0) {
+            void *site = nullptr;
+            for (int t = 0; t < nt; t++) {
+              if (g_fib[t].done) continue;
+              if (!site) site = g_site[t];
+              else if (g_site[t] != site) {
+                fprintf(stderr, "EMU: lanes wait at different barriers (%p, %p)\n", site, g_site[t]);
+                abort();
+              }
+            }
+          }
           alive = nt - ndone;
         }
       }

@@ -1,10 +1,15 @@
 """The library's HIP sources compiled for the CPU (tests/emu) and run under
-AddressSanitizer: every kernel of the plain and fused forms executes on real
-workloads (AMR faces of every kind, cooling, winds, reflective / free-flow /
-periodic boundaries, hydro and MHD), and any out-of-range index into a device
-array aborts the run with its source line.  The emulation itself is checked
-against the oracle: the fused FP64 form must reproduce it bit for bit on
-Orszag-Tang, as it does on the GPU."""
+AddressSanitizer and UBSan's array-bounds checks: every kernel of the plain
+and fused forms executes on real workloads (AMR faces of every kind,
+cooling, winds, reflective / free-flow / periodic boundaries, hydro and
+MHD), and any out-of-range index aborts the run with its source line - into
+a device array (ASan), into a fixed-size array such as an LDS plane or a
+lane's register array (bounds), or outside the ranges the kernels state
+for cell, block and ghost-source indices (WX_EMU_CHECK).  Lanes waiting at
+different barriers abort too, and the fused kernels must give bit-identical
+states when the lanes of every barrier interval run in a shuffled order.
+The emulation itself is checked against the oracle: the fused FP64 form
+must reproduce it bit for bit on Orszag-Tang, as it does on the GPU."""
 import os
 import shutil
 import struct
@@ -32,7 +37,8 @@ CASES = [("ot64", hip.FUSED_FP32, 2), ("sedov", hip.FUSED_FP32, 2), ("blast_refl
 def _build():
     os.makedirs(BUILD, exist_ok=True)
     exe = os.path.join(BUILD, "emu")
-    flags = ["-std=c++20", "-O1", "-g", "-fsanitize=address", "-fno-omit-frame-pointer", "-ffp-contract=off",
+    flags = ["-std=c++20", "-O1", "-g", "-fsanitize=address,bounds", "-fno-sanitize-recover=bounds",
+             "-fno-omit-frame-pointer", "-ffp-contract=off", "-DWX_EMU_CHECKS",
              "-Wno-unknown-pragmas", "-I" + EMU, "-I" + os.path.join(ROOT, "walicxe3d_amd", "csrc"),
              "-I" + os.path.join(ROOT, "include")]
     objs = []
@@ -86,7 +92,8 @@ def emu():
 def _run(emu, args, tmp, **env_extra):
     env = dict(os.environ, ASAN_OPTIONS="detect_stack_use_after_return=0:detect_leaks=0", **env_extra)
     r = subprocess.run([emu] + args, capture_output=True, text=True, timeout=900, env=env, cwd=tmp)
-    assert r.returncode == 0 and "ERROR: AddressSanitizer" not in r.stderr, (r.stdout + r.stderr)[-4000:]
+    assert r.returncode == 0 and "ERROR: AddressSanitizer" not in r.stderr and "runtime error" not in r.stderr, \
+        (r.stdout + r.stderr)[-4000:]
     return r.stdout
 
 
@@ -106,6 +113,24 @@ def test_every_kernel_in_bounds(emu, tmp_path):
         args += [p, "1"]
     out = _run(emu, args, str(tmp_path))
     assert out.count("it ") == len(CASES) + 2, out
+
+
+def test_fused_kernels_do_not_depend_on_lane_order(emu, tmp_path):
+    """The fused stage kernels with the lanes of every barrier interval run
+    in two seeded random orders give the state of the in-order run bit for
+    bit: no lane reads LDS (or device) data that another lane writes in the
+    same interval (a race the GPU would run in some order of its own)."""
+    cases = [("ot64", hip.FUSED_FP32), ("amr_small", hip.FUSED_FP32), ("snr_amr4", hip.FUSED_FP32),
+             ("ot64_amr", hip.FUSED_FP32), ("blast_reflect", hip.FUSED_FP64)]
+    for name, form in cases:
+        outs = []
+        for seed in (None, "1", "77"):
+            p = str(tmp_path / ("%s_%s.bin" % (name, seed)))
+            _write_input(p, W.get(name), form)
+            env = {"EMU_ORDER_SEED": seed} if seed else {}
+            _run(emu, [p, "1"], str(tmp_path), **env)
+            outs.append(np.fromfile(p + ".out"))
+        assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2]), name
 
 
 def test_emulation_reproduces_fused_fp64(emu, tmp_path, oracle):

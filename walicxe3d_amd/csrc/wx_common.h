@@ -18,8 +18,19 @@ enum { BC_REFLECTIVE = 1, BC_FREE_FLOW = 2, BC_PERIODIC = 3 };
 enum { NEIGH_SAME = 1, NEIGH_COARSER = 2, NEIGH_FINER = 3, NEIGH_BOUNDARY = 4 };
 enum { SOLVER_HLL1 = 2, SOLVER_HLL = 3, SOLVER_HLLC = 4, SOLVER_HLLE = 5, SOLVER_HLLD = 6 };
 
+// Index checks of the CPU emulation of the kernels (the emulator build
+// defines WX_EMU_CHECKS): a violated range aborts with its source line.
+// Nothing in the GPU build.
+#ifdef WX_EMU_CHECKS
+void wx_emu_check_failed(const char *what, const char *file, int line);
+#define WX_EMU_CHECK(c) ((c) ? (void)0 : wx_emu_check_failed(#c, __FILE__, __LINE__))
+#else
+#define WX_EMU_CHECK(c) ((void)0)
+#endif
+
 // Index of cell (i,j,k), i,j,k in [-1, 18] (reference bounds nxmin..nxmax)
 __host__ __device__ constexpr inline int cid(int i, int j, int k) {
+  WX_EMU_CHECK(i >= -1 && i <= NC + 2 && j >= -1 && j <= NC + 2 && k >= -1 && k <= NC + 2);
   return ((k + 1) * NB + (j + 1)) * NB + (i + 1);
 }
 
@@ -40,6 +51,7 @@ struct BlockInfo {
   int bid;         // reference block ID
   int coords[3];   // 1-based block coordinates at its level
   Face face[6];
+  int wind;        // 1: a wind source may cover a cell centre of the block or of its two ghost layers
 };
 
 }  // namespace wx

@@ -130,6 +130,7 @@ __global__ void __launch_bounds__(256, WX_MHD_MIN_WAVES) k_mhd(FusedArgs a) {
     // block, level and per-block constants are the same in every lane:
     // scalar registers
     const int b = uni(a.order[bi]);
+    WX_EMU_CHECK(b >= 0 && b < a.nblk);
     const int l = uni(c.info[b].level - 1);
     const double dxl = c.dx[0][l], dyl = c.dx[1][l], dzl = c.dx[2][l];
     const double dtdx = uni(dtp / dxl), dtdy = uni(dtp / dyl), dtdz = uni(dtp / dzl);
