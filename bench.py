@@ -159,16 +159,17 @@ def main():
     # ---- fused FP32 form ----------------------------------------------------
     f = hip.Walicxe3D(w, form=hip.FUSED_FP32, bids=bids)
     f.upload(U0)
-    best = {"grid": 0, "xcd_map": 0, "curve": 0}
+    best = {"grid": 0, "xcd_map": 0, "curve": 0, "nslab": 1}
     if args.no_tune:
         f.set_tuning(0, best["grid"])
         f.set_tuning(1, best["xcd_map"])
         f.set_tuning(2, best["curve"])
+        f.set_tuning(4, best["nslab"])
     else:
         from walicxe3d_amd import autotune
         best, table = autotune.tune(f, steps=3)
-        log("autotune: %s" % ", ".join("grid %d xcd %d curve %d %.3f ms" % (r["grid"], r["xcd_map"], r["curve"],
-                                                                            r["ms"]) for r in table))
+        log("autotune: %s" % ", ".join("grid %d xcd %d curve %d slabs %d %.3f ms" % (
+            r["grid"], r["xcd_map"], r["curve"], r["nslab"], r["ms"]) for r in table))
     f.run(args.warmup)
     remesh = None
     if args.remesh_every > 0:
@@ -248,7 +249,8 @@ def main():
                    "speedup_fused_fp32_over_fused_fp64": fused_zps / v}
         log("fused FP64: %.3f ms/step, %.3e zone-updates/s" % (ms, v))
         if w.nv == 8:
-            v, ms = timed(hip.FUSED_FP32, [(0, best["grid"]), (1, best["xcd_map"]), (2, best["curve"]), (3, 1)])
+            v, ms = timed(hip.FUSED_FP32, [(0, best["grid"]), (1, best["xcd_map"]), (2, best["curve"]), (3, 1),
+                                           (4, best["nslab"])])
             plain32 = {"value": v, "ms_per_step": ms,
                        "form": "fused FP32 with every flux term in FP32 (tuning kind 3 = 1)"}
             log("fused FP32, plain FP32 fluxes: %.3f ms/step, %.3e zone-updates/s" % (ms, v))
@@ -322,7 +324,8 @@ def main():
                             "fused FP32 bulk, FP64 conserved accumulation"), "solver": w.solver,
                    "limiter": w.limiter, "step_hbm_roofline_frac": (fused_zps * step_bytes / zones) / HBM_PEAK,
                    "tuning": {"grid": best["grid"], "xcd_map": best["xcd_map"],
-                              "curve": "hilbert" if best["curve"] else "morton"}, "remesh": remesh},
+                              "curve": "hilbert" if best["curve"] else "morton",
+                              "z_slabs_per_block": best["nslab"]}, "remesh": remesh},
         "roofline": roof,
         "cpu_baseline": cpu,
         "plain_fp64": plain,

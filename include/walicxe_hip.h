@@ -282,7 +282,10 @@ int wx_write_vtk(void *ctx, const char *datadir, int noutput, int npassive, int 
    (1) or not (0), kind 2 = block order along the Morton (0) or Hilbert (1)
    curve, kind 3 = the FP32 MHD kernels' fluxes in delta form (0: FP64
    physical flux plus FP32 Riemann correction, the default) or entirely in
-   FP32 (1: faster, with FP32 rounding of every flux term). */
+   FP32 (1: faster, with FP32 rounding of every flux term), kind 4 = z slabs
+   per block (1, 2, 4 or 8), one workgroup each, in the delta-form MHD
+   kernels without viscosity (more workgroups for meshes of few blocks;
+   results unchanged; no effect elsewhere). */
 int wx_set_tuning(void *ctx, int kind, int value);
 /* Advances `reps` more iterations launching each stage kernel on its own
    between HIP events on the library's stream (fused form; no reference

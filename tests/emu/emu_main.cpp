@@ -51,6 +51,9 @@ static int run_case(const char *path, int nsteps) {
   int rc = wx_init(&cfg, &ctx); if (rc < 0) { printf("init %d %s\n", rc, wx_last_error()); return 1; }
   rc = wx_set_mesh(ctx, nroot, maxlev, dx.data(), nblk, bids.data()); if (rc < 0) { printf("mesh %d %s\n", rc, wx_last_error()); return 1; }
   rc = wx_upload_state(ctx, U.data(), WX_LAYOUT_BLOCKS, 0, nullptr); if (rc < 0) { printf("upload %d %s\n", rc, wx_last_error()); return 1; }
+  if (getenv("EMU_NSLAB") && cfg.form != WX_FORM_PLAIN_FP64) {
+    rc = wx_set_tuning(ctx, 4, atoi(getenv("EMU_NSLAB"))); if (rc < 0) { printf("slabs %d %s\n", rc, wx_last_error()); return 1; }
+  }
   const char *vtkdir = getenv("EMU_VTK_DIR");
   const int vtkunits = getenv("EMU_VTK_UNITS") ? atoi(getenv("EMU_VTK_UNITS")) : 2;
   if (vtkdir && wx_write_vtk(ctx, vtkdir, 0, 1, vtkunits) < 0) { printf("vtk %s\n", wx_last_error()); return 1; }

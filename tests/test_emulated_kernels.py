@@ -133,6 +133,21 @@ def test_fused_kernels_do_not_depend_on_lane_order(emu, tmp_path):
         assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2]), name
 
 
+def test_z_slabs_leave_the_state_unchanged(emu, tmp_path):
+    """The FP32 MHD kernels with each block split into 2, 4 and 8 z slabs
+    (one workgroup each) stay in bounds and give the whole-block state bit
+    for bit (a periodic and a two-level mesh)."""
+    for name in ("ot64", "ot64_amr"):
+        outs = []
+        for nslab in ("1", "2", "4", "8"):
+            p = str(tmp_path / ("%s_%s.bin" % (name, nslab)))
+            _write_input(p, W.get(name), hip.FUSED_FP32)
+            _run(emu, [p, "2"], str(tmp_path), EMU_NSLAB=nslab)
+            outs.append(np.fromfile(p + ".out"))
+        for o in outs[1:]:
+            assert np.array_equal(o, outs[0]), name
+
+
 def test_emulation_reproduces_fused_fp64(emu, tmp_path, oracle):
     w = W.get("ot64")
     p = str(tmp_path / "ot64.bin")

@@ -372,9 +372,11 @@ def test_fused_remesh_recaptures_graph():
 @pytest.mark.parametrize("name", ["amr_small", "ot64", "ot64_amr"])
 def test_fused_launch_and_order_choices_leave_results_unchanged(name):
     """The autotuner's choices (persistent grid, XCD ranges, Morton or
-    Hilbert block order) change only which workgroup updates which block
-    when: every block's update reads the same data, and the dt reduction is
-    an exact minimum, so the state is bit-identical."""
+    Hilbert block order, z slabs per block) change only which workgroup
+    updates which cells when: every cell's update reads the same data (a
+    slab's bottom z face is computed by both slabs that share it, with the
+    same code and inputs), and the dt reduction is an exact minimum, so the
+    state is bit-identical."""
     from walicxe3d_amd import hip
     w = W.get(name)
     bids = np.asarray(w.blocks(), dtype=np.int32)
@@ -383,11 +385,13 @@ def test_fused_launch_and_order_choices_leave_results_unchanged(name):
     out = []
     # persistent grids smaller than the block count: each workgroup updates
     # several blocks in turn
-    for grid, xcd, curve in ((0, 0, 0), (32, 1, 1), (0, 1, 1), (48, 0, 0)):
+    for grid, xcd, curve, nslab in ((0, 0, 0, 1), (32, 1, 1, 1), (0, 1, 1, 1), (48, 0, 0, 1), (0, 0, 0, 2),
+                                    (0, 1, 0, 4), (0, 0, 1, 8), (40, 0, 0, 4)):
         g = gpu(w, hip.FUSED_FP32, bids)
         g.set_tuning(0, grid)
         g.set_tuning(1, xcd)
         g.set_tuning(2, curve)
+        g.set_tuning(4, nslab)
         g.upload(U0)
         g.run(4)
         out.append((g.dt.value, g.download(prim=False)))

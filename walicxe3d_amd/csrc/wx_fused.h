@@ -40,6 +40,7 @@ struct FusedState {
   int xcd_map = 1;           // XCD-aware block order
   int curve = 0;             // block order along the Morton (0) or Hilbert (1) curve
   int mhd_plain32 = 0;       // FP32 MHD: plain FP32 fluxes (1) instead of the delta form (0)
+  int nslab = 1;             // FP32 MHD without viscosity: z slabs per block, one workgroup each
   int nroot[3] = {1, 1, 1};  // root blocks per axis (Hilbert curve order)
   hipGraph_t graph = nullptr;      // one step from the current U
   hipGraphExec_t exec = nullptr;

@@ -200,6 +200,7 @@ static FusedArgs make_args(FusedState &f, const DevCfg &c) {
   a.order = f.order;
   a.step = f.step;
   a.nblk = f.nblk;
+  a.nslab = 1;
   return a;
 }
 
@@ -264,8 +265,12 @@ static void launch_stage_t(const FusedArgs &a, int grid, int nv, int solver, boo
 
 static void launch_stage(FusedState &f, const DevCfg &c, hipStream_t st, int stage) {
   FusedArgs a = make_args(f, c);
-  int grid = f.grid > 0 ? std::min(f.grid, f.nblk) : f.nblk;
   bool visc = c.visc_eta != 0.0;
+  // z slabs: the delta-form MHD kernels without viscosity (the viscous
+  // epilogue works one plane behind and needs whole columns)
+  if (!f.fp64 && c.nv == 8 && !f.mhd_plain32 && !visc) a.nslab = f.nslab;
+  const int items = f.nblk * a.nslab;
+  int grid = f.grid > 0 ? std::min(f.grid, items) : items;
   if (f.fp64) launch_stage_t<double, double>(a, grid, c.nv, c.pd.solver, visc, stage, 0, st);
   else launch_stage_t<float, float>(a, grid, c.nv, c.pd.solver, visc, stage, f.mhd_plain32, st);
 }
@@ -377,12 +382,19 @@ int fused_time_stages(FusedState &f, const DevCfg &c, hipStream_t st, int reps) 
 // kind 1: XCD-aware block order (1) or one range for all XCDs (0)
 // kind 2: block order along the Morton (0) or the Hilbert (1) curve
 // kind 3: FP32 MHD fluxes in delta form (0) or plain FP32 (1)
+// kind 4: z slabs per block (1, 2, 4, 8), one workgroup each, in the
+//         delta-form MHD kernels without viscosity (no effect elsewhere)
 int fused_set_tuning(FusedState &f, int kind, int value) {
-  if (kind < 0 || kind > 3) return 1;
+  if (kind < 0 || kind > 4) return 1;
   if ((kind == 2 || kind == 3) && (value < 0 || value > 1)) return 1;
+  if (kind == 4 && value != 1 && value != 2 && value != 4 && value != 8) return 1;
   drop_graphs(f);
   if (kind == 3) {
     f.mhd_plain32 = value;
+    return 0;
+  }
+  if (kind == 4) {
+    f.nslab = value;
     return 0;
   }
   if (kind == 0) {

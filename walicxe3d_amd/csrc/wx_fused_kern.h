@@ -26,6 +26,7 @@ struct FusedArgs {
   const int *order;
   StepState *step;
   int nblk;
+  int nslab;         // z slabs per block, one workgroup each (FP32 MHD kernels without viscosity; else 1)
 };
 
 template <typename RF> __device__ __forceinline__ const Phys<RF> &phys(const DevCfg &c);

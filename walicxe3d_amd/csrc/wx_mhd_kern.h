@@ -126,7 +126,16 @@ __global__ void __launch_bounds__(256, WX_MHD_MIN_WAVES) k_mhd(FusedArgs a) {
   const double icv = uni(1.0 / c.pd.cv);
   double dtmin = 1.0e30;
 
-  for (int bi = blockIdx.x; bi < a.nblk; bi += gridDim.x) {
+  // Work items: block bi of the order list, z slab s of it (planes
+  // s L + 1 .. (s + 1) L, L = NC / nslab; the viscous kernels run whole
+  // blocks, nslab = 1).  A slab starts one plane early, at its bottom z face,
+  // which the slab below computes too, with the same code and inputs.
+  const int nslab = uni(a.nslab);
+  WX_EMU_CHECK(nslab >= 1 && NC % nslab == 0 && (!SV || nslab == 1));
+  const int L = NC / nslab;
+  for (int item = blockIdx.x; item < a.nblk * nslab; item += gridDim.x) {
+    const int bi = item / nslab;
+    const int k0 = (item - bi * nslab) * L + 1, k1 = k0 + L - 1;
     // block, level and per-block constants are the same in every lane:
     // scalar registers
     const int b = uni(a.order[bi]);
@@ -145,8 +154,8 @@ __global__ void __launch_bounds__(256, WX_MHD_MIN_WAVES) k_mhd(FusedArgs a) {
     double unew_prev[SV ? NV : 1], unew_prev2[SV ? NV : 1];  // UP_new at planes k-1, k-2 (viscosity)
 #pragma unroll
     for (int w = 1; w < NW; w++) {
-      const int kk = w - 2;                                  // S2: -1, 0, 1 ; S1: -1, 0
-      if (!S2 && kk < 0) continue;
+      const int kk = k0 - 3 + w;                             // S2: k0-2, k0-1, k0 ; S1: k0-1 (w = 2)
+      if (!S2 && w == 1) continue;
       if constexpr (S2) load_cell<float, NV, true, true>(a, UPin, b, i, j, kk, ucol);
       else load_cell<double, NV, false>(a, a.U, b, i, j, kk, ucol);
       if constexpr (SV) {
@@ -157,7 +166,7 @@ __global__ void __launch_bounds__(256, WX_MHD_MIN_WAVES) k_mhd(FusedArgs a) {
       prims64(icv, ucol, wq[w]);
     }
 
-    for (int k = 0; k <= NC; k++) {
+    for (int k = k0 - 1; k <= k1; k++) {
 #pragma unroll
       for (int w = 0; w < NW - 1; w++)
 #pragma unroll
@@ -184,7 +193,7 @@ __global__ void __launch_bounds__(256, WX_MHD_MIN_WAVES) k_mhd(FusedArgs a) {
         }
         face_flux<2, SOLVER>(c, wl, wr, fzc);
       }
-      if (k == 0) {
+      if (k == k0 - 1) {
 #pragma unroll
         for (int v = 0; v < NV; v++) fzp[v] = fzc[v];
         continue;
