@@ -17,6 +17,7 @@ def test_slab_choices_follow_the_kernels():
     assert autotune.slab_choices(fake("mhd_hlle", hip.FUSED_FP32)) == (1,)     # viscosity: whole columns
     assert autotune.slab_choices(fake("sedov", hip.FUSED_FP32)) == (1,)        # hydro kernels
     assert autotune.slab_choices(fake("ot64", hip.FUSED_FP64)) == (1,)
+    assert autotune.slab_choices(fake("ot64", hip.PLAIN_FP64)) == (1,)
 
 
 def test_candidates_cover_every_dimension():

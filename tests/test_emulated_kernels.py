@@ -136,12 +136,14 @@ def test_fused_kernels_do_not_depend_on_lane_order(emu, tmp_path):
 def test_z_slabs_leave_the_state_unchanged(emu, tmp_path):
     """The FP32 MHD kernels with each block split into 2, 4 and 8 z slabs
     (one workgroup each) stay in bounds and give the whole-block state bit
-    for bit (a periodic and a two-level mesh)."""
-    for name in ("ot64", "ot64_amr"):
+    for bit (a periodic and a two-level mesh).  Elsewhere the setting has
+    no effect (viscous MHD, hydro)."""
+    for name, form in (("ot64", hip.FUSED_FP32), ("ot64_amr", hip.FUSED_FP32), ("mhd_hlle", hip.FUSED_FP32),
+                       ("sedov", hip.FUSED_FP32)):
         outs = []
         for nslab in ("1", "2", "4", "8"):
             p = str(tmp_path / ("%s_%s.bin" % (name, nslab)))
-            _write_input(p, W.get(name), hip.FUSED_FP32)
+            _write_input(p, W.get(name), form)
             _run(emu, [p, "2"], str(tmp_path), EMU_NSLAB=nslab)
             outs.append(np.fromfile(p + ".out"))
         for o in outs[1:]:

@@ -765,34 +765,6 @@ int wx_finalize(void *ctx) {
   return WX_OK;
 }
 
-// BlockInfo::wind: whether wind_state can return true for any cell centre
-// of block b with i, j, k in [-1, 18] (the block and two ghost layers).
-// The distance from a source centre to the box of those centres (computed
-// as wind_state computes the centres) bounds the distance to each of them
-// from below; a relative margin of 1e-9 keeps the test conservative under
-// rounding, so a block flagged 0 is one where wind_state returns false for
-// every cell.
-static void mark_wind_blocks(Ctx *c) {
-  const int nw = c->cfg.nwind;
-  for (int b = 0; b < c->nblk; b++) {
-    BlockInfo &B = c->info[b];
-    B.wind = 0;
-    const int l = B.level - 1;
-    for (int w = 0; w < nw && !B.wind; w++) {
-      const double *wp = &c->winds[8 * w];
-      double d2 = 0.0;
-      for (int a = 0; a < 3; a++) {
-        const double dxl = c->dxyz[a * c->maxlev + l];
-        const double lo = ((B.coords[a] - 1) * NC + (-2) + 0.5) * dxl * c->cfg.l_sc;
-        const double hi = ((B.coords[a] - 1) * NC + (NC + 1) + 0.5) * dxl * c->cfg.l_sc;
-        const double e = wp[a] < lo ? lo - wp[a] : (wp[a] > hi ? wp[a] - hi : 0.0);
-        d2 += e * e;
-      }
-      if (std::sqrt(d2) < wp[3] * (1.0 + 1e-9)) B.wind = 1;
-    }
-  }
-}
-
 // wx_set_mesh; `pre`, when given, holds the face table of this block list
 // already built (check_mesh with the context's nroot, maxlev and boundaries),
 // which is taken over instead of built again
@@ -813,7 +785,6 @@ static int set_mesh_impl(Ctx *c, const int nroot[3], int maxlev, const double *d
     rc = build_topology(c, bids);
   }
   if (rc != WX_OK) return rc;
-  mark_wind_blocks(c);
   size_t n = (size_t)nblk * c->cfg.nv * NCELL;
   HIPCHK(hipMalloc(&c->d_info, sizeof(BlockInfo) * nblk));
   HIPCHK(hipMemcpy(c->d_info, c->info.data(), sizeof(BlockInfo) * nblk, hipMemcpyHostToDevice));

@@ -51,7 +51,6 @@ struct BlockInfo {
   int bid;         // reference block ID
   int coords[3];   // 1-based block coordinates at its level
   Face face[6];
-  int wind;        // 1: a wind source may cover a cell centre of the block or of its two ghost layers
 };
 
 }  // namespace wx

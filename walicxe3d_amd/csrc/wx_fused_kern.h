@@ -186,7 +186,7 @@ __device__ __forceinline__ void load_cell(const FusedArgs &a, const T *W, int b,
       }
     }
   }
-  if (a.c.nwind > 0 && (WIN || !in) && a.c.info[b].wind) {
+  if (a.c.nwind > 0 && (WIN || !in)) {
     double w[MAXV];
     if (wind_state(a.c, b, i, j, k, w)) {
 #pragma unroll
@@ -355,21 +355,25 @@ __global__ void __launch_bounds__(256, (NV == 8 && !S2) ? WX_MHD_STAGE1_MIN_WAVE
   const int tid = threadIdx.x;
   const int ti = tid & 15, tj = tid >> 4;
   const int i = ti + 1, j = tj + 1;
-  const double dt = uni(a.step->dt);
+  const double dt = NV == 8 ? uni(a.step->dt) : a.step->dt;
   const double dtp = S2 ? dt : dt / 2.0;
   double dtmin = 1.0e30;
 
   for (int bi = blockIdx.x; bi < a.nblk; bi += gridDim.x) {
     // the block and its level are the same in every lane (and every lane is
-    // active here, at the top of the block loop): they are read into scalar
-    // registers, so that block addresses and per-level values stay scalar
-    const int b = uni(a.order[bi]);
+    // active here, at the top of the block loop): in the MHD kernels they are
+    // read into scalar registers, so that block addresses and per-level
+    // values stay scalar
+    const int b = NV == 8 ? uni(a.order[bi]) : a.order[bi];
     WX_EMU_CHECK(b >= 0 && b < a.nblk);
-    const int l = uni(c.info[b].level - 1);
+    const int l = NV == 8 ? uni(c.info[b].level - 1) : c.info[b].level - 1;
     const double dxl = c.dx[0][l], dyl = c.dx[1][l], dzl = c.dx[2][l];
-    const double dtdx = uni(dtp / dxl), dtdy = uni(dtp / dyl), dtdz = uni(dtp / dzl);
-    const RF idx2 = uni((RF)(1.0 / (2.0 * dxl))), idy2 = uni((RF)(1.0 / (2.0 * dyl))),
-             idz2 = uni((RF)(1.0 / (2.0 * dzl)));
+    double dtdx = dtp / dxl, dtdy = dtp / dyl, dtdz = dtp / dzl;
+    RF idx2 = (RF)(1.0 / (2.0 * dxl)), idy2 = (RF)(1.0 / (2.0 * dyl)), idz2 = (RF)(1.0 / (2.0 * dzl));
+    if constexpr (NV == 8) {
+      dtdx = uni(dtdx); dtdy = uni(dtdy); dtdz = uni(dtdz);
+      idx2 = uni(idx2); idy2 = uni(idy2); idz2 = uni(idz2);
+    }
 
     // column window of primitives; at iteration k (after the shift)
     // S2: win = planes {k-1, k, k+1, k+2};  S1: win = {k-1, k, k+1}
@@ -502,7 +506,7 @@ __global__ void __launch_bounds__(256, (NV == 8 && !S2) ? WX_MHD_STAGE1_MIN_WAVE
         double ub[MAXV];
 #pragma unroll
         for (int v = 0; v < NV; v++) ub[v] = Ub[v * NINT];
-        if (c.nwind > 0 && c.info[b].wind) {
+        if (c.nwind > 0) {
           double w[MAXV];
           if (wind_state(c, b, i, j, k, w))
 #pragma unroll
