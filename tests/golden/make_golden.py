@@ -216,7 +216,8 @@ def make_unit(name, ops):
         elif op in (5, 6, 7, 8):
             pl = rand_prims(rng, n, nv, w.mhd)
             pr = rand_prims(rng, n, nv, w.mhd)
-            # a share of near-identical and strongly supersonic pairs covers every branch
+            # a share of near-identical and strongly supersonic pairs reaches every region
+            # of each solver (tests/test_golden_coverage.py counts them)
             pr[::6] = pl[::6] * (1 + 1e-3 * rng.normal(size=(pr[::6].shape)))
             pl[1::6, 1] += 8.0
             pr[1::6, 1] += 8.0
