@@ -70,12 +70,15 @@ static void to_sched() {
   emu_switch(&F.sp, g_sched_sp);
   __sanitizer_finish_switch_fiber(F.fake, nullptr, nullptr);
 }
-// call site of each lane's current barrier: every lane of a round must wait
-// at the same __syncthreads (a barrier that lanes reach from different
-// places of the kernel is divergence the GPU does not tolerate)
-static std::vector<void *> g_site;
-__attribute__((noinline)) void __syncthreads() {
-  g_site[g_cur] = __builtin_return_address(0);
+// source position (file, line) of each lane's current barrier: every lane
+// of a round must wait at the same __syncthreads of the source (a barrier
+// that lanes reach from different places of the kernel is divergence the GPU
+// does not tolerate; copies of one call that the compiler makes are the
+// same barrier)
+struct EmuSite { const char *file; int line; };
+static std::vector<EmuSite> g_site;
+void emu_syncthreads(const char *file, int line) {
+  g_site[g_cur] = EmuSite{file, line};
   to_sched();
 }
 double emu_shfl_xor(double v, int off) {
@@ -103,7 +106,7 @@ void emu_run_grid(dim3 grid, dim3 block, std::function<void()> body) {
   blockDim = block;
   int nt = block.x * block.y * block.z;
   g_fib.resize(nt);
-  g_site.assign(nt, nullptr);
+  g_site.assign(nt, EmuSite{nullptr, 0});
   // EMU_ORDER_SEED: run the lanes of each barrier interval in a seeded
   // random order instead of lane order (a kernel whose result depends on
   // that order reads shared data another lane writes in the same interval)
@@ -151,12 +154,13 @@ void emu_run_grid(dim3 grid, dim3 block, std::function<void()> body) {
           }
           if (ndone > 0 && nbar > 0) { fprintf(stderr, "EMU: divergent barrier (%d done, %d waiting)\n", ndone, nbar); abort(); }
           if (nbar > 0) {
-            void *site = nullptr;
+            const EmuSite *site = nullptr;
             for (int t = 0; t < nt; t++) {
               if (g_fib[t].done) continue;
-              if (!site) site = g_site[t];
-              else if (g_site[t] != site) {
-                fprintf(stderr, "EMU: lanes wait at different barriers (%p, %p)\n", site, g_site[t]);
+              if (!site) site = &g_site[t];
+              else if (g_site[t].line != site->line || strcmp(g_site[t].file, site->file) != 0) {
+                fprintf(stderr, "EMU: lanes wait at different barriers (%s:%d, %s:%d)\n", site->file, site->line,
+                        g_site[t].file, g_site[t].line);
                 abort();
               }
             }

@@ -63,6 +63,12 @@ inline hipError_t hipSetDevice(int) { return hipSuccess; }
 inline hipError_t hipDeviceSynchronize() { return hipSuccess; }
 template <typename T> inline hipError_t hipMalloc(T **p, size_t n) { *p = (T *)malloc(n ? n : 1); return *p ? hipSuccess : 2; }
 inline hipError_t hipFree(void *p) { free(p); return hipSuccess; }
+enum { hipHostMallocDefault = 0 };
+template <typename T> inline hipError_t hipHostMalloc(T **p, size_t n, unsigned) {
+  *p = (T *)malloc(n ? n : 1);
+  return *p ? hipSuccess : 2;
+}
+inline hipError_t hipHostFree(void *p) { free(p); return hipSuccess; }
 inline hipError_t hipMemcpy(void *d, const void *s, size_t n, hipMemcpyKind) { memcpy(d, s, n); return hipSuccess; }
 
 void emu_record_or_run(hipStream_t st, std::function<void()> f);
@@ -99,7 +105,8 @@ inline hipError_t hipGraphDestroy(hipGraph_t g) { delete g; return hipSuccess; }
 inline hipError_t hipGraphExecDestroy(hipGraphExec_t g) { delete g; return hipSuccess; }
 
 // ---- device intrinsics -------------------------------------------------
-void __syncthreads();
+void emu_syncthreads(const char *file, int line);
+#define __syncthreads() emu_syncthreads(__FILE__, __LINE__)
 double emu_shfl_xor(double v, int off);
 inline double __shfl_xor(double v, int off) { return emu_shfl_xor(v, off); }
 inline long long __double_as_longlong(double d) { long long r; memcpy(&r, &d, 8); return r; }

@@ -108,6 +108,13 @@ struct Ctx {
   double *d_ghosts = nullptr;        // [blk][rho, p][6][256]
   double *d_mark = nullptr;          // [blk] largest gradient (wx_mark_blocks)
   FusedState fused;
+  // staging of the reference's Fortran layout (fortran_stage_*): a chunk of
+  // [v, k] planes in the order [plane][j][i][slot], on the device and in
+  // pinned host memory, and the slot -> block map
+  double *d_stage = nullptr, *h_stage = nullptr;
+  size_t stage_cap = 0;              // doubles
+  int *d_slotmap = nullptr;
+  int slotmap_cap = 0;
 };
 
 void free_mesh(Ctx *c) {
@@ -312,25 +319,139 @@ __global__ void k_gather(int nv, int nblk, const double *src, double *dst) {
   dst[t] = src[bv * NCELL + cid(i, j, k)];
 }
 
-int to_fortran_or_back(Ctx *c, const double *in, double *out, bool to_blocks, int nbmax, const int *lb) {
-  // Fortran U(nbmax, nv, -1:18, -1:18, -1:18): slot fastest
-  int nv = c->cfg.nv;
+int launch_check(const char *what);
+
+// The reference's U(nbmax, nv, -1:18, -1:18, -1:18) keeps the slot index
+// fastest, so one block's values are nbmax * nv doubles apart: the device
+// reorders a chunk of [v, k] planes between the block layout [blk][nv][16^3]
+// and [plane][j][i][slot], and the host moves, for each interior cell and
+// variable, the contiguous runs of occupied slots (one memcpy per run)
+// between that staging and the Fortran array.
+constexpr size_t STAGE_CHUNK = size_t(1) << 23;   // doubles per chunk (64 MB)
+
+__global__ void k_blocks_to_stage(int nv, int nbmax, int plane0, int nplanes, const int *slot2blk,
+                                  const double *src, double *stage) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)nplanes * NC * NC * nbmax) return;
+  const int s = (int)(t % nbmax);
+  const size_t rest = t / nbmax;
+  const int cell = (int)(rest % (NC * NC));                 // (j - 1) * NC + (i - 1)
+  const int pl = plane0 + (int)(rest / (NC * NC));          // v * NC + (k - 1)
+  const int v = pl / NC, k0 = pl % NC;
+  const int b = slot2blk[s];
+  stage[t] = b >= 0 ? src[((size_t)b * nv + v) * NINT + (size_t)k0 * NC * NC + cell] : 0.0;
+}
+
+__global__ void k_stage_to_blocks(int nv, int nbmax, int plane0, int nplanes, const int *slot2blk,
+                                  const double *stage, double *dst) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)nplanes * NC * NC * nbmax) return;
+  const int s = (int)(t % nbmax);
+  const size_t rest = t / nbmax;
+  const int cell = (int)(rest % (NC * NC));
+  const int pl = plane0 + (int)(rest / (NC * NC));
+  const int v = pl / NC, k0 = pl % NC;
+  const int b = slot2blk[s];
+  if (b >= 0) dst[((size_t)b * nv + v) * NINT + (size_t)k0 * NC * NC + cell] = stage[t];
+}
+
+// slot -> block map on the device and the runs of occupied slots; staging
+// buffers grown to one chunk
+static int stage_prepare(Ctx *c, int nbmax, const int *lb, std::vector<std::pair<int, int>> &runs, int &planes) {
+  std::vector<int> map(nbmax);
+  runs.clear();
   for (int s = 0; s < nbmax; s++) {
-    const int b = lb[s] == -1 ? -1 : c->index.find(lb[s]);
-    if (b < 0) continue;
-    for (int v = 0; v < nv; v++)
-      for (int k = 1; k <= NC; k++)
-        for (int j = 1; j <= NC; j++)
-          for (int i = 1; i <= NC; i++) {
-            size_t fo = (size_t)s + (size_t)nbmax * (v + (size_t)nv * ((i + 1) + NB * ((j + 1) + (size_t)NB * (k + 1))));
-            size_t bo = (((size_t)b * nv + v) * NINT) + ((k - 1) * NC + (j - 1)) * NC + (i - 1);
-            if (to_blocks)
-              out[bo] = in[fo];
-            else
-              out[fo] = in[bo];
-          }
+    map[s] = lb[s] == -1 ? -1 : c->index.find(lb[s]);
+    if (map[s] < 0) continue;
+    if (!runs.empty() && runs.back().second == s) runs.back().second = s + 1;
+    else runs.push_back({s, s + 1});
   }
+  const size_t plane = (size_t)NC * NC * nbmax;
+  planes = (int)std::max<size_t>(1, std::min<size_t>((size_t)c->cfg.nv * NC, STAGE_CHUNK / plane));
+  const size_t need = plane * planes;
+  if (need > c->stage_cap) {
+    hipFree(c->d_stage);
+    hipHostFree(c->h_stage);
+    c->d_stage = nullptr;
+    c->h_stage = nullptr;
+    c->stage_cap = 0;
+    HIPCHK(hipMalloc(&c->d_stage, need * sizeof(double)));
+    HIPCHK(hipHostMalloc(&c->h_stage, need * sizeof(double), hipHostMallocDefault));
+    c->stage_cap = need;
+  }
+  if (nbmax > c->slotmap_cap) {
+    hipFree(c->d_slotmap);
+    c->d_slotmap = nullptr;
+    c->slotmap_cap = 0;
+    HIPCHK(hipMalloc(&c->d_slotmap, nbmax * sizeof(int)));
+    c->slotmap_cap = nbmax;
+  }
+  HIPCHK(hipMemcpy(c->d_slotmap, map.data(), nbmax * sizeof(int), hipMemcpyHostToDevice));
   return WX_OK;
+}
+
+// offset of U(s, v, i, j, k) in the Fortran array (0-based slot s)
+static inline size_t fortran_offset(int nbmax, int nv, int s, int v, int i, int j, int k) {
+  return (size_t)s + (size_t)nbmax * (v + (size_t)nv * ((i + 1) + NB * ((j + 1) + (size_t)NB * (k + 1))));
+}
+
+// device block layout -> Fortran array on the host (interior cells of the
+// occupied slots; nothing else of the array is written)
+static int fortran_stage_out(Ctx *c, const double *d_src, double *out, int nbmax, const int *lb) {
+  const int nv = c->cfg.nv;
+  std::vector<std::pair<int, int>> runs;
+  int planes;
+  int rc = stage_prepare(c, nbmax, lb, runs, planes);
+  if (rc != WX_OK) return rc;
+  for (int p0 = 0; p0 < nv * NC; p0 += planes) {
+    const int np = std::min(planes, nv * NC - p0);
+    const size_t n = (size_t)np * NC * NC * nbmax;
+    hipLaunchKernelGGL(k_blocks_to_stage, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->st, nv, nbmax, p0, np,
+                       c->d_slotmap, d_src, c->d_stage);
+    HIPCHK(hipMemcpyAsync(c->h_stage, c->d_stage, n * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    for (int pl = 0; pl < np; pl++) {
+      const int v = (p0 + pl) / NC, k = (p0 + pl) % NC + 1;
+      for (int j = 1; j <= NC; j++)
+        for (int i = 1; i <= NC; i++) {
+          const double *row = c->h_stage + ((size_t)(pl * NC + (j - 1)) * NC + (i - 1)) * nbmax;
+          double *dst = out + fortran_offset(nbmax, nv, 0, v, i, j, k);
+          for (const auto &r : runs) std::memcpy(dst + r.first, row + r.first, (r.second - r.first) * sizeof(double));
+        }
+    }
+  }
+  return launch_check("Fortran layout");
+}
+
+// Fortran array on the host -> device block layout (blocks of the context
+// with no slot in lb are zero)
+static int fortran_stage_in(Ctx *c, const double *in, double *d_dst, int nbmax, const int *lb) {
+  const int nv = c->cfg.nv;
+  std::vector<std::pair<int, int>> runs;
+  int planes;
+  int rc = stage_prepare(c, nbmax, lb, runs, planes);
+  if (rc != WX_OK) return rc;
+  HIPCHK(hipMemsetAsync(d_dst, 0, (size_t)c->nblk * nv * NINT * sizeof(double), c->st));
+  for (int p0 = 0; p0 < nv * NC; p0 += planes) {
+    const int np = std::min(planes, nv * NC - p0);
+    const size_t n = (size_t)np * NC * NC * nbmax;
+    // the previous chunk's copy out of h_stage is done before it is refilled
+    HIPCHK(hipStreamSynchronize(c->st));
+    for (int pl = 0; pl < np; pl++) {
+      const int v = (p0 + pl) / NC, k = (p0 + pl) % NC + 1;
+      for (int j = 1; j <= NC; j++)
+        for (int i = 1; i <= NC; i++) {
+          double *row = c->h_stage + ((size_t)(pl * NC + (j - 1)) * NC + (i - 1)) * nbmax;
+          const double *src = in + fortran_offset(nbmax, nv, 0, v, i, j, k);
+          for (const auto &r : runs) std::memcpy(row + r.first, src + r.first, (r.second - r.first) * sizeof(double));
+        }
+    }
+    HIPCHK(hipMemcpyAsync(c->d_stage, c->h_stage, n * sizeof(double), hipMemcpyHostToDevice, c->st));
+    hipLaunchKernelGGL(k_stage_to_blocks, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->st, nv, nbmax, p0, np,
+                       c->d_slotmap, c->d_stage, d_dst);
+  }
+  HIPCHK(hipStreamSynchronize(c->st));
+  return launch_check("Fortran layout");
 }
 
 int launch_check(const char *what) {
@@ -760,6 +881,9 @@ int wx_finalize(void *ctx) {
   hipFree(c->d_scal);
   hipFree(c->d_tabT);
   hipFree(c->d_tabC);
+  hipFree(c->d_stage);
+  hipHostFree(c->h_stage);
+  hipFree(c->d_slotmap);
   hipStreamDestroy(c->st);
   delete c;
   return WX_OK;
@@ -846,19 +970,19 @@ int wx_upload_state(void *ctx, const double *u, int layout, int nbmax, const int
   if (!c->d_info) return fail(WX_ESTATE, "no mesh");
   int nv = c->cfg.nv;
   size_t ni = (size_t)c->nblk * nv * NINT;
-  std::vector<double> tmp;
-  const double *src = u;
   if (layout == WX_LAYOUT_FORTRAN) {
     if (!lb || nbmax < 1) return fail(WX_EINVAL, "Fortran layout needs nbmax and localBlocks");
-    tmp.assign(ni, 0.0);
-    to_fortran_or_back(c, u, tmp.data(), true, nbmax, lb);
-    src = tmp.data();
   } else if (layout != WX_LAYOUT_BLOCKS) {
     return fail(WX_EINVAL, "unknown layout");
   }
   DevBuf<double> d_in;
   HIPCHK(d_in.alloc(ni));
-  HIPCHK(hipMemcpyAsync(d_in.p, src, ni * sizeof(double), hipMemcpyHostToDevice, c->st));
+  if (layout == WX_LAYOUT_FORTRAN) {
+    int rc = fortran_stage_in(c, u, d_in.p, nbmax, lb);
+    if (rc != WX_OK) return rc;
+  } else {
+    HIPCHK(hipMemcpyAsync(d_in.p, u, ni * sizeof(double), hipMemcpyHostToDevice, c->st));
+  }
   return upload_device(c, d_in.p);
 }
 
@@ -870,19 +994,19 @@ int wx_upload_prims(void *ctx, const double *p, int layout, int nbmax, const int
   if (is_fused(c)) return fail(WX_EINVAL, "plain-form entry point");
   int nv = c->cfg.nv;
   size_t ni = (size_t)c->nblk * nv * NINT;
-  std::vector<double> tmp;
-  const double *src = p;
   if (layout == WX_LAYOUT_FORTRAN) {
     if (!lb || nbmax < 1) return fail(WX_EINVAL, "Fortran layout needs nbmax and localBlocks");
-    tmp.assign(ni, 0.0);
-    to_fortran_or_back(c, p, tmp.data(), true, nbmax, lb);
-    src = tmp.data();
   } else if (layout != WX_LAYOUT_BLOCKS) {
     return fail(WX_EINVAL, "unknown layout");
   }
   DevBuf<double> d_in;
   HIPCHK(d_in.alloc(ni));
-  HIPCHK(hipMemcpyAsync(d_in.p, src, ni * sizeof(double), hipMemcpyHostToDevice, c->st));
+  if (layout == WX_LAYOUT_FORTRAN) {
+    rc = fortran_stage_in(c, p, d_in.p, nbmax, lb);
+    if (rc != WX_OK) return rc;
+  } else {
+    HIPCHK(hipMemcpyAsync(d_in.p, p, ni * sizeof(double), hipMemcpyHostToDevice, c->st));
+  }
   size_t total = (size_t)c->nblk * nv * NCELL;
   const double *in = d_in.p;
   hipLaunchKernelGGL(k_scatter, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, c->st, nv, c->nblk, in, c->dP, 0);
@@ -903,27 +1027,31 @@ int wx_download_state(void *ctx, double *u, double *prim, int layout, int nbmax,
   int nv = c->cfg.nv;
   size_t ni = (size_t)c->nblk * nv * NINT;
   DevBuf<double> d_out;
-  HIPCHK(d_out.alloc(ni));
-  double *out = d_out.p;
-  std::vector<double> tmp(layout == WX_LAYOUT_FORTRAN ? ni : 0);
   for (int w = 0; w < 2; w++) {
     double *dst = w == 0 ? u : prim;
     if (!dst) continue;
-    const double *from = out;
-    if (is_fused(c)) {
+    const double *from;
+    if (is_fused(c) && w == 0) {
       // the fused state is already in the block layout of the interface
-      if (w == 0)
-        from = c->fused.U;
-      else if (fused_prims(c->fused, c->dc, out, c->st))
-        return fail(WX_EHIP, fused_error());
+      from = c->fused.U;
     } else {
-      hipLaunchKernelGGL(k_gather, dim3((unsigned)((ni + 255) / 256)), dim3(256), 0, c->st, nv, c->nblk,
-                         w == 0 ? c->dU : c->dP, out);
+      if (!d_out.p) HIPCHK(d_out.alloc(ni));
+      double *out = d_out.p;
+      if (is_fused(c)) {
+        if (fused_prims(c->fused, c->dc, out, c->st)) return fail(WX_EHIP, fused_error());
+      } else {
+        const double *g = w == 0 ? c->dU : c->dP;
+        hipLaunchKernelGGL(k_gather, dim3((unsigned)((ni + 255) / 256)), dim3(256), 0, c->st, nv, c->nblk, g, out);
+      }
+      from = out;
     }
-    double *h = layout == WX_LAYOUT_FORTRAN ? tmp.data() : dst;
-    HIPCHK(hipMemcpyAsync(h, from, ni * sizeof(double), hipMemcpyDeviceToHost, c->st));
-    HIPCHK(hipStreamSynchronize(c->st));
-    if (layout == WX_LAYOUT_FORTRAN) to_fortran_or_back(c, tmp.data(), dst, false, nbmax, lb);
+    if (layout == WX_LAYOUT_FORTRAN) {
+      rc = fortran_stage_out(c, from, dst, nbmax, lb);
+      if (rc != WX_OK) return rc;
+    } else {
+      HIPCHK(hipMemcpyAsync(dst, from, ni * sizeof(double), hipMemcpyDeviceToHost, c->st));
+      HIPCHK(hipStreamSynchronize(c->st));
+    }
   }
   return launch_check("download");
 }
