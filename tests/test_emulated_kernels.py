@@ -10,6 +10,7 @@ different barriers abort too, and the fused kernels must give bit-identical
 states when the lanes of every barrier interval run in a shuffled order.
 The emulation itself is checked against the oracle: the fused FP64 form
 must reproduce it bit for bit on Orszag-Tang, as it does on the GPU."""
+import fcntl
 import os
 import shutil
 import struct
@@ -35,7 +36,14 @@ CASES = [("ot64", hip.FUSED_FP32, 2), ("sedov", hip.FUSED_FP32, 2), ("blast_refl
 
 
 def _build():
+    # one build at a time: parallel test workers share the build folder
     os.makedirs(BUILD, exist_ok=True)
+    with open(os.path.join(BUILD, ".lock"), "w") as lock:
+        fcntl.flock(lock, fcntl.LOCK_EX)
+        return _build_locked()
+
+
+def _build_locked():
     exe = os.path.join(BUILD, "emu")
     flags = ["-std=c++20", "-O1", "-g", "-fsanitize=address,bounds", "-fno-sanitize-recover=bounds",
              "-fno-omit-frame-pointer", "-ffp-contract=off", "-DWX_EMU_CHECKS",

@@ -105,7 +105,8 @@ template <int NV, typename RF> __device__ __forceinline__ void rot(int d, RF *v)
 // before boundary() re-imposes the sources; source/godunov.f90:77-83).
 // DEL: W holds the half-step state as FP32 increments over U^n (a.U), and
 // the value of an element is U^n + increment, formed in FP64.
-template <typename T, int NV, bool WIN, bool DEL = false>
+// WINDS = false leaves the wind code out (kernels for meshes without sources).
+template <typename T, int NV, bool WIN, bool DEL = false, bool WINDS = true>
 __device__ __forceinline__ void load_cell(const FusedArgs &a, const T *W, int b, int i, int j, int k, double *u) {
   bool in = (unsigned)(i - 1) < NC && (unsigned)(j - 1) < NC && (unsigned)(k - 1) < NC;
   if (in) {
@@ -186,7 +187,7 @@ __device__ __forceinline__ void load_cell(const FusedArgs &a, const T *W, int b,
       }
     }
   }
-  if (a.c.nwind > 0 && (WIN || !in)) {
+  if (WINDS && a.c.nwind > 0 && (WIN || !in)) {
     double w[MAXV];
     if (wind_state(a.c, b, i, j, k, w)) {
 #pragma unroll

@@ -99,7 +99,7 @@ __device__ __forceinline__ void face_flux(const DevCfg &c, double *wl, double *w
 #define WX_MHD_MIN_WAVES 2
 #endif
 
-template <bool S2, bool VISC, int SOLVER, int LIM>
+template <bool S2, bool VISC, int SOLVER, int LIM, bool WIND>
 __global__ void __launch_bounds__(256, WX_MHD_MIN_WAVES) k_mhd(FusedArgs a) {
   constexpr int NV = 8;
   constexpr int H = S2 ? 2 : 1;            // halo depth of the stencil
@@ -153,35 +153,34 @@ __global__ void __launch_bounds__(256, WX_MHD_MIN_WAVES) k_mhd(FusedArgs a) {
     double fzp[NV];                                          // z flux (k-1 | k)
     double unew_prev[SV ? NV : 1], unew_prev2[SV ? NV : 1];  // UP_new at planes k-1, k-2 (viscosity)
 #pragma unroll
-    for (int w = 1; w < NW; w++) {
-      const int kk = k0 - 3 + w;                             // S2: k0-2, k0-1, k0 ; S1: k0-1 (w = 2)
-      if (!S2 && w == 1) continue;
-      if constexpr (S2) load_cell<float, NV, true, true>(a, UPin, b, i, j, kk, ucol);
-      else load_cell<double, NV, false>(a, a.U, b, i, j, kk, ucol);
-      if constexpr (SV) {
-        if (kk == 0)
+    for (int w = 0; w < NW; w++)
 #pragma unroll
-          for (int v = 0; v < NV; v++) ughost_lo[v] = ucol[v];
-      }
-      prims64(icv, ucol, wq[w]);
-    }
+      for (int v = 0; v < NV; v++) wq[w][v] = 0.0;
 
-    for (int k = k0 - 1; k <= k1; k++) {
+    // Iteration k loads plane k + NW - 2 into the window.  The first
+    // iterations only fill it (S2: planes k0-2 .. k0, S1: plane k0-1), so
+    // that one call site of load_cell, with its ghost paths, serves every
+    // plane: the kernel's code stays small.
+    for (int k = k0 - (S2 ? 4 : 2); k <= k1; k++) {
 #pragma unroll
       for (int w = 0; w < NW - 1; w++)
 #pragma unroll
         for (int v = 0; v < NV; v++) wq[w][v] = wq[w + 1][v];
       {
         const int kn = S2 ? k + 2 : k + 1;
-        if constexpr (S2) load_cell<float, NV, true, true>(a, UPin, b, i, j, kn, ucol);
-        else load_cell<double, NV, false>(a, a.U, b, i, j, kn, ucol);
+        if constexpr (S2) load_cell<float, NV, true, true, WIND>(a, UPin, b, i, j, kn, ucol);
+        else load_cell<double, NV, false, false, WIND>(a, a.U, b, i, j, kn, ucol);
         if constexpr (SV) {
+          if (kn == 0)
+#pragma unroll
+            for (int v = 0; v < NV; v++) ughost_lo[v] = ucol[v];
           if (kn == NC + 1)
 #pragma unroll
             for (int v = 0; v < NV; v++) ughost_hi[v] = ucol[v];
         }
         prims64(icv, ucol, wq[NW - 1]);
       }
+      if (k < k0 - 1) continue;
       // z flux (k | k+1): one code path for every plane, the bottom face included
       double fzc[NV];
       {
@@ -212,8 +211,8 @@ __global__ void __launch_bounds__(256, WX_MHD_MIN_WAVES) k_mhd(FusedArgs a) {
         if (part == 0) { hi = side ? NC + g : 1 - g; hj = aa; }
         else { hj = side ? NC + g : 1 - g; hi = aa; }
         double uh[NV], wh[NV];
-        if constexpr (S2) load_cell<float, NV, true, true>(a, UPin, b, hi, hj, k, uh);
-        else load_cell<double, NV, false>(a, a.U, b, hi, hj, k, uh);
+        if constexpr (S2) load_cell<float, NV, true, true, WIND>(a, UPin, b, hi, hj, k, uh);
+        else load_cell<double, NV, false, false, WIND>(a, a.U, b, hi, hj, k, uh);
         prims64(icv, uh, wh);
 #pragma unroll
         for (int v = 0; v < NV; v++) sw[v][(hj - 1 + H) * WPD + (hi - 1 + H)] = wh[v];
@@ -348,18 +347,26 @@ __global__ void __launch_bounds__(256, WX_MHD_MIN_WAVES) k_mhd(FusedArgs a) {
   }
 }
 
+template <int SOLVER, bool WIND>
+inline void launch_mhd_w(const FusedArgs &a, int grid, bool visc, int stage, hipStream_t st) {
+  constexpr int MM = 1;  // LIMITER_MINMOD (source/constants.f90)
+  const bool mm = a.c.pf.limiter == MM && !WIND;
+  if (stage == 1)
+    hipLaunchKernelGGL((k_mhd<false, false, SOLVER, LIM_RUNTIME, WIND>), dim3(grid), dim3(256), 0, st, a);
+  else if (visc)
+    hipLaunchKernelGGL((k_mhd<true, true, SOLVER, LIM_RUNTIME, WIND>), dim3(grid), dim3(256), 0, st, a);
+  else if (mm)
+    hipLaunchKernelGGL((k_mhd<true, false, SOLVER, MM, false>), dim3(grid), dim3(256), 0, st, a);
+  else
+    hipLaunchKernelGGL((k_mhd<true, false, SOLVER, LIM_RUNTIME, WIND>), dim3(grid), dim3(256), 0, st, a);
+}
+
+// meshes with wind sources take the kernels that carry the wind code (every
+// limiter at run time); the others leave it out
 template <int SOLVER>
 inline void launch_mhd(const FusedArgs &a, int grid, bool visc, int stage, hipStream_t st) {
-  constexpr int MM = 1;  // LIMITER_MINMOD (source/constants.f90)
-  const bool mm = a.c.pf.limiter == MM;
-  if (stage == 1)
-    hipLaunchKernelGGL((k_mhd<false, false, SOLVER, LIM_RUNTIME>), dim3(grid), dim3(256), 0, st, a);
-  else if (visc)
-    hipLaunchKernelGGL((k_mhd<true, true, SOLVER, LIM_RUNTIME>), dim3(grid), dim3(256), 0, st, a);
-  else if (mm)
-    hipLaunchKernelGGL((k_mhd<true, false, SOLVER, MM>), dim3(grid), dim3(256), 0, st, a);
-  else
-    hipLaunchKernelGGL((k_mhd<true, false, SOLVER, LIM_RUNTIME>), dim3(grid), dim3(256), 0, st, a);
+  if (a.c.nwind > 0) launch_mhd_w<SOLVER, true>(a, grid, visc, stage, st);
+  else launch_mhd_w<SOLVER, false>(a, grid, visc, stage, st);
 }
 
 }  // namespace wx
