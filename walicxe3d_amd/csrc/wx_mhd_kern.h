@@ -2,7 +2,8 @@
 //
 // Same block-per-workgroup structure as k_stage (wx_fused_kern.h): thread
 // (i, j) owns a z-column and marches k; the column's cells sit in a
-// register window, the plane k and its x/y face halo in LDS.  The
+// register window, the plane k and its x/y face halo in LDS (the second
+// stage, short of registers, keeps plane k only there).  The
 // arithmetic is split so that FP32 rounding never touches an O(1) part of
 // a face flux:
 //   - primitives of every cell in FP64 (from the FP64 state, or in the
@@ -126,6 +127,29 @@ __global__ void __launch_bounds__(256, WX_MHD_MIN_WAVES) k_mhd(FusedArgs a) {
   const double icv = uni(1.0 / c.pd.cv);
   double dtmin = 1.0e30;
 
+  // x/y face halo of a plane, H layers on each of the 4 sides: lanes
+  // tid < NH load one cell each.  The first stage issues these loads at the
+  // top of the plane (their latency overlaps the z face); the second, whose
+  // registers are full there, after the first barrier.
+  constexpr int NH = 4 * H * NC;
+  constexpr bool EARLY_HALO = !S2;
+  // second stage: plane k of the column window lives in this lane's slot of
+  // sw (where the x/y faces read it) rather than in registers, from the
+  // point where it is stored until it is read back as plane k-1
+  constexpr bool LW = S2;
+  const int own = (j - 1 + H) * WPD + (i - 1 + H);
+  const bool hth = tid < NH;
+  // halo cell (hi, hj) of lane tid, layer g
+  auto halo_cell = [&](int &hi, int &hj, int &g) {
+    const int part = tid / (2 * H * NC);
+    const int r = tid % (2 * H * NC);
+    const int side = r / (H * NC);
+    g = (r % (H * NC)) / NC + 1;
+    const int aa = r % NC + 1;
+    if (part == 0) { hi = side ? NC + g : 1 - g; hj = aa; }
+    else { hj = side ? NC + g : 1 - g; hi = aa; }
+  };
+
   // Work items: block bi of the order list, z slab s of it (planes
   // s L + 1 .. (s + 1) L, L = NC / nslab; the viscous kernels run whole
   // blocks, nslab = 1).  A slab starts one plane early, at its bottom z face,
@@ -162,10 +186,21 @@ __global__ void __launch_bounds__(256, WX_MHD_MIN_WAVES) k_mhd(FusedArgs a) {
     // that one call site of load_cell, with its ghost paths, serves every
     // plane: the kernel's code stays small.
     for (int k = k0 - (S2 ? 4 : 2); k <= k1; k++) {
+      if constexpr (LW) {
+        // plane k-1 from this lane's slot of the plane store, where the
+        // previous iteration left it
 #pragma unroll
-      for (int w = 0; w < NW - 1; w++)
+        for (int v = 0; v < NV; v++) wq[0][v] = sw[v][own];
 #pragma unroll
-        for (int v = 0; v < NV; v++) wq[w][v] = wq[w + 1][v];
+        for (int w = 1; w < NW - 1; w++)
+#pragma unroll
+          for (int v = 0; v < NV; v++) wq[w][v] = wq[w + 1][v];
+      } else {
+#pragma unroll
+        for (int w = 0; w < NW - 1; w++)
+#pragma unroll
+          for (int v = 0; v < NV; v++) wq[w][v] = wq[w + 1][v];
+      }
       {
         const int kn = S2 ? k + 2 : k + 1;
         if constexpr (S2) load_cell<float, NV, true, true, WIND>(a, UPin, b, i, j, kn, ucol);
@@ -180,6 +215,28 @@ __global__ void __launch_bounds__(256, WX_MHD_MIN_WAVES) k_mhd(FusedArgs a) {
         }
         prims64(icv, ucol, wq[NW - 1]);
       }
+      if constexpr (LW) {
+        // while the window fills, the slot takes plane k here (later after
+        // the first barrier, with the rest of the plane)
+        if (k < k0)
+#pragma unroll
+          for (int v = 0; v < NV; v++) sw[v][own] = wq[1][v];
+      }
+      double wh[NV];                       // halo cell primitives
+      float uhf[SV ? NV : 1];              // halo cell state (viscosity ring)
+      auto halo_load = [&]() {
+        int hi, hj, hg;
+        halo_cell(hi, hj, hg);
+        double uh[NV];
+        if constexpr (S2) load_cell<float, NV, true, true, WIND>(a, UPin, b, hi, hj, k, uh);
+        else load_cell<double, NV, false, false, WIND>(a, a.U, b, hi, hj, k, uh);
+        prims64(icv, uh, wh);
+        if constexpr (SV) {
+#pragma unroll
+          for (int v = 0; v < NV; v++) uhf[v] = (float)uh[v];
+        }
+      };
+      if (EARLY_HALO && hth && k >= k0) halo_load();
       if (k < k0 - 1) continue;
       // z flux (k | k+1): one code path for every plane, the bottom face included
       double fzc[NV];
@@ -200,26 +257,17 @@ __global__ void __launch_bounds__(256, WX_MHD_MIN_WAVES) k_mhd(FusedArgs a) {
       __syncthreads();   // the previous plane's LDS readers are done
       // own primitives of plane k into LDS; halo threads add the face halo
 #pragma unroll
-      for (int v = 0; v < NV; v++) sw[v][(j - 1 + H) * WPD + (i - 1 + H)] = wq[1][v];
-      if (tid < 4 * H * NC) {
-        const int part = tid / (2 * H * NC);
-        const int r = tid % (2 * H * NC);
-        const int side = r / (H * NC);
-        const int g = (r % (H * NC)) / NC + 1;
-        const int aa = r % NC + 1;
-        int hi, hj;
-        if (part == 0) { hi = side ? NC + g : 1 - g; hj = aa; }
-        else { hj = side ? NC + g : 1 - g; hi = aa; }
-        double uh[NV], wh[NV];
-        if constexpr (S2) load_cell<float, NV, true, true, WIND>(a, UPin, b, hi, hj, k, uh);
-        else load_cell<double, NV, false, false, WIND>(a, a.U, b, hi, hj, k, uh);
-        prims64(icv, uh, wh);
+      for (int v = 0; v < NV; v++) sw[v][own] = wq[1][v];
+      if (hth) {
+        if (!EARLY_HALO) halo_load();
+        int hi, hj, hg;
+        halo_cell(hi, hj, hg);
 #pragma unroll
         for (int v = 0; v < NV; v++) sw[v][(hj - 1 + H) * WPD + (hi - 1 + H)] = wh[v];
         if constexpr (SV) {
-          if (g == 1)
+          if (hg == 1)
 #pragma unroll
-            for (int v = 0; v < NV; v++) sun[k & 1][v][hj][hi] = (float)uh[v];
+            for (int v = 0; v < NV; v++) sun[k & 1][v][hj][hi] = uhf[v];
         }
       }
       __syncthreads();
@@ -272,10 +320,16 @@ __global__ void __launch_bounds__(256, WX_MHD_MIN_WAVES) k_mhd(FusedArgs a) {
                     dtdz * (fzp[v] - fzc[v]);
         if (c.eight_wave) {
           // Powell source (source/sources.f90:38-95) from the stage primitives
-          const int q = (j - 1 + H) * WPD + (i - 1 + H);
+          const int q = own;
           const double divB = (sw[5][q + 1] - sw[5][q - 1]) * idx2 + (sw[6][q + WPD] - sw[6][q - WPD]) * idy2 +
                               (wq[2][7] - wq[0][7]) * idz2;
+          double ppl[NV];
           const double *pp = wq[1];
+          if constexpr (LW) {
+#pragma unroll
+            for (int v = 0; v < NV; v++) ppl[v] = sw[v][q];
+            pp = ppl;
+          }
           unew[1] += dtp * (-divB * pp[5]);
           unew[2] += dtp * (-divB * pp[6]);
           unew[3] += dtp * (-divB * pp[7]);
