@@ -274,6 +274,34 @@ def test_fused_fp32_error_vs_fp64_oracle(oracle, name, nsteps, tol, tol_small):
     g.close()
 
 
+@pytest.mark.parametrize("visc", [5.0e-3, 0.0])
+def test_fused_mhd_with_wind_sources(oracle, visc):
+    """MHD meshes with wind sources take the FP32 stage kernels that carry
+    the wind code: in a cell a source covers, each stage's update starts from
+    the wind state, as the reference's boundary() imposes it before each
+    stage.  Against the oracle after 3 steps, viscous and inviscid, per
+    variable."""
+    from walicxe3d_amd import hip
+    w = dataclasses.replace(W.get("mhd_winds4"), visc_eta=visc)
+    bids = np.asarray(w.blocks(), dtype=np.int32)
+    U0 = ic_in_order(w, bids)
+    say("case fused MHD winds, visc", visc)
+    g = gpu(w, hip.FUSED_FP32, bids)
+    g.upload(U0)
+    g.run(3)
+    o = oracle.Sim(w, bids)
+    o.set_state(U0)
+    for _ in range(3):
+        o.step()
+    ug, _ = g.download()
+    uo, _ = o.get_state()
+    e = rel_l1(ug, uo)
+    say("  rel L1 per variable", [float("%.3e" % x) for x in e])
+    assert np.all(np.isfinite(ug))
+    assert np.all(e <= 1e-5), e
+    g.close()
+
+
 def drift(U0, u):
     """Change of the domain totals of mass, momentum (3), energy and field (3)
     relative to the L1 size of that quantity (momentum and field measured by

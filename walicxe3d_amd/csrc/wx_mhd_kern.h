@@ -313,9 +313,21 @@ __global__ void __launch_bounds__(256, WX_MHD_MIN_WAVES) k_mhd(FusedArgs a) {
         double ub[NV];
 #pragma unroll
         for (int v = 0; v < NV; v++) ub[v] = Ub[v * NINT];
+        // base of the update: U^n, or the wind state in a cell that a source
+        // covers (each stage's boundary() imposes the sources through
+        // userBoundary: source/godunov.f90:79,141, source/boundary.f90:61)
+        double u0[NV];
+#pragma unroll
+        for (int v = 0; v < NV; v++) u0[v] = ub[v];
+        if constexpr (WIND) {
+          double w[MAXV];
+          if (c.nwind > 0 && wind_state(c, b, i, j, k, w))
+#pragma unroll
+            for (int v = 0; v < NV; v++) u0[v] = w[v];
+        }
 #pragma unroll
         for (int v = 0; v < NV; v++)
-          unew[v] = ub[v] + dtdx * (sfx[v][tj * (NC + 1) + ti] - sfx[v][tj * (NC + 1) + ti + 1]) +
+          unew[v] = u0[v] + dtdx * (sfx[v][tj * (NC + 1) + ti] - sfx[v][tj * (NC + 1) + ti + 1]) +
                     dtdy * (sfy[v][ti * (NC + 1) + tj] - sfy[v][ti * (NC + 1) + tj + 1]) +
                     dtdz * (fzp[v] - fzc[v]);
         if (c.eight_wave) {
@@ -342,7 +354,7 @@ __global__ void __launch_bounds__(256, WX_MHD_MIN_WAVES) k_mhd(FusedArgs a) {
           // the half-step state as FP32 increments over U^n
           float *UPo = UPout + (size_t)b * NV * NINT + lin(i, j, k);
 #pragma unroll
-          for (int v = 0; v < NV; v++) UPo[v * NINT] = (float)(unew[v] - ub[v]);
+          for (int v = 0; v < NV; v++) UPo[v * NINT] = (float)(unew[v] - ub[v]);   // over the stored U^n
         }
       }
 #pragma unroll

@@ -344,6 +344,13 @@ _reg(_w(name="winds4", mhd=False, solver="hll", limiter="minmod", gamma_expr="5.
         nroot=(1, 1, 1), maxlev=5, base_level=2, phys=(10.0 * PC, 10.0 * PC, 10.0 * PC), ic="winds", seed=5,
         nsteps=5, cooling=True, refine_rule="winds", **_PHYS_UNITS,
         description="three spherical wind sources on 4 AMR levels, cooling, HLL"))
+# MHD with wind sources (the stage kernels that carry the wind code), HLLD;
+# no reference build: the oracle checks it
+_reg(_w(name="mhd_winds4", mhd=True, solver="hlld", limiter="minmod", gamma_expr="5.0/3.0",
+        gamma=5.0 / 3.0, cfl=0.4, visc_eta=5.0e-3, eight_wave=True, bc=FREE6,
+        nroot=(1, 1, 1), maxlev=5, base_level=2, phys=(10.0 * PC, 10.0 * PC, 10.0 * PC), ic="winds", seed=5,
+        nsteps=5, cooling=True, refine_rule="winds", oracle=False, **_PHYS_UNITS,
+        description="three spherical wind sources on 4 AMR levels in a uniform field, cooling, MHD, HLLD"))
 # C5 full size (GPU only): winds, root 8^3 at level 4, 5 levels (4..8), ~32k blocks; HLL and HLLC
 for _solver, _suffix in (("hll", ""), ("hllc", "_hllc")):
     _reg(_w(name="winds32k" + _suffix, mhd=False, solver=_solver, limiter="minmod", gamma_expr="5.0/3.0",
@@ -560,6 +567,11 @@ def block_ic(w: Workload, m: Mesh, bid: int) -> np.ndarray:
             vol = 4.0 / 3.0 * math.pi * rsn ** 3
             esn = 1.0e51 / (w.p_sc * w.l_sc ** 3)
             p = np.where(r <= rsn, (w.gamma - 1.0) * esn / vol, p)
+        if w.mhd:
+            # a uniform field in the x-y plane, plasma beta 1.6 at the mean
+            # ambient state (the wind sources impose B = 0 inside)
+            b = math.sqrt(1.0 * w.d_sc / (w.mu0 * AMU) * KB * T / w.p_sc)
+            return _prim_to_cons(w, rho, zeros, zeros, zeros, p, zeros + b, zeros + 0.5 * b, zeros)
         return _prim_to_cons(w, rho, zeros, zeros, zeros, p)
     raise ValueError(w.ic)
 
