@@ -274,18 +274,18 @@ def test_fused_fp32_error_vs_fp64_oracle(oracle, name, nsteps, tol, tol_small):
     g.close()
 
 
-@pytest.mark.parametrize("visc", [5.0e-3, 0.0])
-def test_fused_mhd_with_wind_sources(oracle, visc):
+@pytest.mark.parametrize("visc,solver", [(5.0e-3, "hlld"), (0.0, "hlld"), (0.0, "hlle")])
+def test_fused_mhd_with_wind_sources(oracle, visc, solver):
     """MHD meshes with wind sources take the FP32 stage kernels that carry
     the wind code: in a cell a source covers, each stage's update starts from
     the wind state, as the reference's boundary() imposes it before each
-    stage.  Against the oracle after 3 steps, viscous and inviscid, per
-    variable."""
+    stage.  Against the oracle after 3 steps, viscous and inviscid, HLLD and
+    HLLE, per variable."""
     from walicxe3d_amd import hip
-    w = dataclasses.replace(W.get("mhd_winds4"), visc_eta=visc)
+    w = dataclasses.replace(W.get("mhd_winds4"), visc_eta=visc, solver=solver)
     bids = np.asarray(w.blocks(), dtype=np.int32)
     U0 = ic_in_order(w, bids)
-    say("case fused MHD winds, visc", visc)
+    say("case fused MHD winds, visc", visc, solver)
     g = gpu(w, hip.FUSED_FP32, bids)
     g.upload(U0)
     g.run(3)

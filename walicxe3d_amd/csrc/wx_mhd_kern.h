@@ -81,6 +81,15 @@ template <int D> __device__ __forceinline__ void rot8(double *v) {
   }
 }
 
+// rot8<1> where d holds (the y direction), per lane: selects, no branch
+__device__ __forceinline__ void rot8_if(bool d, double *v) {
+  const double v1 = v[1], v2 = v[2], v5 = v[5], v6 = v[6];
+  v[1] = d ? v2 : v1;
+  v[2] = d ? v1 : v2;
+  v[5] = d ? v6 : v5;
+  v[6] = d ? v5 : v6;
+}
+
 // Flux across one face along D from its two face states (unrotated) in the
 // delta form of wx_delta.h; a face near the degenerate HLLD star state
 // takes the one-code-path HLLD in FP32 (hlld_flux_ill)
@@ -271,12 +280,9 @@ __global__ void __launch_bounds__(256, WX_MHD_MIN_WAVES) k_mhd(FusedArgs a) {
         }
       }
       __syncthreads();
-      // x faces of plane k: s = 0..15 of every row, then in a second round
-      // (lanes 0..15) the 16 faces s = 16; one code path for all of them
-      for (int t = tid; t < NXF; t += 256) {
-        const int s = t < 256 ? (t & 15) : NC;
-        const int row = t < 256 ? (t >> 4) : t - 256;
-        const int base = (row + H) * WPD + (s - 1 + H);     // cell s
+      // x faces s = 0..15 of plane k, one per lane (row tj, face ti)
+      {
+        const int base = (tj + H) * WPD + (ti - 1 + H);       // cell s = ti
         double wl[NV], wr[NV], ff[NV];
 #pragma unroll
         for (int v = 0; v < NV; v++) {
@@ -286,14 +292,13 @@ __global__ void __launch_bounds__(256, WX_MHD_MIN_WAVES) k_mhd(FusedArgs a) {
         }
         face_flux<0, SOLVER>(c, wl, wr, ff);
 #pragma unroll
-        for (int v = 0; v < NV; v++) sfx[v][row * (NC + 1) + s] = ff[v];
+        for (int v = 0; v < NV; v++) sfx[v][tj * (NC + 1) + ti] = ff[v];
       }
-      // y faces, the second round on lanes 64..79 (another wave than the x
-      // faces' second round)
-      for (int t = (tid + 192) & 255; t < NXF; t += 256) {
-        const int s = t < 256 ? (t & 15) : NC;
-        const int col = t < 256 ? (t >> 4) : t - 256;
-        const int base = (s - 1 + H) * WPD + (col + H);     // cell s
+      // y faces s = 0..15, one per lane
+      {
+        const int t = (tid + 192) & 255;
+        const int sy = t & 15, col = t >> 4;
+        const int base = (sy - 1 + H) * WPD + (col + H);      // cell s
         double wl[NV], wr[NV], ff[NV];
 #pragma unroll
         for (int v = 0; v < NV; v++) {
@@ -303,7 +308,30 @@ __global__ void __launch_bounds__(256, WX_MHD_MIN_WAVES) k_mhd(FusedArgs a) {
         }
         face_flux<1, SOLVER>(c, wl, wr, ff);
 #pragma unroll
-        for (int v = 0; v < NV; v++) sfy[v][col * (NC + 1) + s] = ff[v];
+        for (int v = 0; v < NV; v++) sfy[v][col * (NC + 1) + sy] = ff[v];
+      }
+      // the 16 x and the 16 y faces s = 16 together on the last 32 lanes (x
+      // on the first 16, y on the others): one solve per lane, the states
+      // rotated by lane (selects) into the x-aligned order of face_flux<0>
+      if (tid >= 256 - 2 * NC) {
+        const int e = tid - (256 - 2 * NC);
+        const bool d = e >= NC;                                // y face
+        const int aa = d ? e - NC : e;                         // row or column
+        const int st = d ? WPD : 1;
+        const int base = d ? (NC - 1 + H) * WPD + (aa + H) : (aa + H) * WPD + (NC - 1 + H);
+        double wl[NV], wr[NV], ff[NV];
+#pragma unroll
+        for (int v = 0; v < NV; v++) {
+          const double *x = &sw[v][base];
+          if constexpr (S2) rec64<LIM>(P, x[-st], x[0], x[st], x[2 * st], wl[v], wr[v]);
+          else { wl[v] = x[0]; wr[v] = x[st]; }
+        }
+        rot8_if(d, wl);
+        rot8_if(d, wr);
+        face_flux<0, SOLVER>(c, wl, wr, ff);
+        rot8_if(d, ff);
+#pragma unroll
+        for (int v = 0; v < NV; v++) (d ? sfy[v] : sfx[v])[aa * (NC + 1) + NC] = ff[v];
       }
       __syncthreads();
       // conservative update of cell (i, j, k) in FP64
