@@ -157,18 +157,22 @@ def test_fused_kernels_do_not_depend_on_lane_order(emu, tmp_path):
 def test_z_slabs_leave_the_state_unchanged(emu, tmp_path):
     """The FP32 MHD kernels with each block split into 2, 4 and 8 z slabs
     (one workgroup each) stay in bounds and give the whole-block state bit
-    for bit (a periodic and a two-level mesh).  Elsewhere the setting has
-    no effect (viscous MHD, hydro)."""
-    for name, form in (("ot64", hip.FUSED_FP32), ("ot64_amr", hip.FUSED_FP32), ("mhd_hlle", hip.FUSED_FP32),
-                       ("sedov", hip.FUSED_FP32)):
+    for bit (a periodic and a two-level mesh, and the kernels that carry the
+    wind code on a mesh with wind sources).  Elsewhere the setting has no
+    effect (viscous MHD, hydro)."""
+    import dataclasses
+    cases = [(W.get(n), "1 2 4 8", "2") for n in ("ot64", "ot64_amr", "mhd_hlle", "sedov")]
+    # the kernels that carry the wind code (inviscid: in slabs), one step
+    cases.append((dataclasses.replace(W.get("mhd_winds4"), visc_eta=0.0), "1 8", "1"))
+    for w, slabs, steps in cases:
         outs = []
-        for nslab in ("1", "2", "4", "8"):
-            p = str(tmp_path / ("%s_%s.bin" % (name, nslab)))
-            _write_input(p, W.get(name), form)
-            _run(emu, [p, "2"], str(tmp_path), EMU_NSLAB=nslab)
+        for nslab in slabs.split():
+            p = str(tmp_path / ("%s_%s.bin" % (w.name, nslab)))
+            _write_input(p, w, hip.FUSED_FP32)
+            _run(emu, [p, steps], str(tmp_path), EMU_NSLAB=nslab)
             outs.append(np.fromfile(p + ".out"))
         for o in outs[1:]:
-            assert np.array_equal(o, outs[0]), name
+            assert np.array_equal(o, outs[0]), w.name
 
 
 def test_emulation_reproduces_fused_fp64(emu, tmp_path, oracle):
