@@ -90,6 +90,19 @@ const char *wx_version(void);
 int wx_set_mesh(void *ctx, const int nroot[3], int maxlev, const double *dxyz, int nblk,
                 const int *bids);
 
+/* A plane wind (imposePlaneWind, source/winds.f90:250-340, as a problem's
+   userBoundary, source/boundary.f90:61, would call it after its spherical
+   sources): at every boundary call, cells k = 16..18 (the top interior plane
+   and the two ghost layers, every i and j) of each block at the domain's top
+   edge take the state rho [g cm^-3], temp [K], mu, velocity vel [cm s^-1]
+   along +x (plane PLANE_LEFT=1), -x (PLANE_RIGHT=2), +y (PLANE_FRONT=3),
+   -y (PLANE_BACK=4), +z (PLANE_BOTTOM=5) or -z (PLANE_TOP=6).  As in the
+   reference, the cells are those of the top edge whatever the plane.
+   plane = 0 removes it.  Plain FP64 form and hydro only (the reference leaves
+   the field of the state unset): other contexts get WX_EINVAL.  No
+   counterpart call in the reference: it replaces the user module's call. */
+int wx_set_plane_wind(void *ctx, int plane, double rho, double vel, double temp, double mu);
+
 /* State transfer.  Upload sets U's interior, zeroes its ghosts and computes
    PRIM (updatePrims, source/hydro_core.f90:43-59).  For WX_LAYOUT_FORTRAN
    pass nbmax = nbMaxProc and local_blocks = localBlocks; slots whose bID is

@@ -142,6 +142,8 @@ def build(name, gpu=False, nprocs=1, amr=False):
              "-ffp-contract=off", "-I" + MPI_INC, "-J", bdir, "-w"]
     if w.mhd:
         flags.append("-DBFIELD")
+    if gpu:
+        flags.append("-DWX_DROPIN")
     objs = []
     for where, fn in (gpu_sources() if gpu else SOURCES):
         if where == "ref":
@@ -194,7 +196,7 @@ def main(argv):
     return 0
 
 
-DROPIN_WORKLOADS = ["sedov", "ot64", "amr_small", "blast_reflect"]
+DROPIN_WORKLOADS = ["sedov", "ot64", "amr_small", "blast_reflect", "plane_wind"]
 # (workload, MPI ranks): 16 ranks = the host cores one GPU of the box comes with
 BASELINE_BUILDS = [("ot64", 16)]
 # workloads whose fixtures include the reference's mesh adaptation

@@ -44,6 +44,10 @@ program wx_oracle_harness
   use init
   use hydro_core
   use hydro_solver, only : hydroSolver
+#ifdef WX_DROPIN
+  use hydro_solver, only : hydro_gpu_set_plane_wind
+  use userconds, only : have_pwind, pwind
+#endif
   use coolingModule, only : cooling, find_coef
   use load_balance, only : loadBalance
   use amr, only : meshlevel, markByPhysical, admesh, getHilbertKey
@@ -81,6 +85,11 @@ program wx_oracle_harness
   call loadBalance ()
   call updatePrims ()
   if (nextout == 0) nextout = 1
+#ifdef WX_DROPIN
+  ! the GPU drop-in does not call userBoundary: a problem registers its
+  ! plane wind with it instead
+  if (have_pwind) call hydro_gpu_set_plane_wind (pwind%plane, pwind%rho, pwind%vel, pwind%temp, pwind%mu)
+#endif
 
   ! multi-rank runs (timing): the master writes the step log only
   if (rank == master) then

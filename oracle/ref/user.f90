@@ -16,7 +16,9 @@
 !                                 when the partner block lives elsewhere, and
 !                                 spherical wind sources through the
 !                                 reference's own imposeSphericalWind
-!                                 (source/winds.f90:123)
+!                                 (source/winds.f90:123), then a plane
+!                                 wind through its imposePlaneWind
+!                                 (source/winds.f90:250)
 !
 ! Multi-rank runs (nProcs > 1): the root blocks start on the master
 ! (source/init.f90:766-786), so the master refines and fills every block and
@@ -26,7 +28,9 @@
 ! File layout (little endian, stream):
 !   int32 magic (0x43495857), nv, nrefine, nblocks, nwind
 !   int32 refine(nrefine)
-!   float64 wind(8, nwind)   xc,yc,zc,radius,mdot,vinf,temp,mu (cgs)
+!   float64 wind(8, nwind)   xc,yc,zc,radius,mdot,vinf,temp,mu (cgs); a
+!                            record with radius < 0 is the plane wind:
+!                            plane,0,0,-1,rho,vel,temp,mu
 !   nblocks x { int32 bID ; float64 u(16,16,16,nv) with i fastest, then j,k,
 !               then variable }
 !
@@ -36,13 +40,15 @@ module userconds
 
   use parameters
   use globals
-  use winds, only : spherical_wind_type, imposeSphericalWind
+  use winds, only : spherical_wind_type, imposeSphericalWind, plane_wind_type, imposePlaneWind
   use utils, only : find
   implicit none
 
   integer, parameter :: maxwinds = 8
   integer :: nwinds = 0
   type(spherical_wind_type) :: wind_src(maxwinds)
+  logical :: have_pwind = .false.
+  type(plane_wind_type) :: pwind
 
 contains
 
@@ -81,20 +87,30 @@ contains
     end if
     call syncBlockLists ()
 
-    nwinds = nw
+    nwinds = 0
     do n=1,nw
       read (77) wrec
-      wind_src(n)%xc = wrec(1)
-      wind_src(n)%yc = wrec(2)
-      wind_src(n)%zc = wrec(3)
-      wind_src(n)%vx = 0.0
-      wind_src(n)%vy = 0.0
-      wind_src(n)%vz = 0.0
-      wind_src(n)%radius = wrec(4)
-      wind_src(n)%mdot = wrec(5)
-      wind_src(n)%vinf = wrec(6)
-      wind_src(n)%temp = wrec(7)
-      wind_src(n)%mu = wrec(8)
+      if (wrec(4) < 0.0) then
+        have_pwind = .true.
+        pwind%plane = nint(wrec(1))
+        pwind%rho = wrec(5)
+        pwind%vel = wrec(6)
+        pwind%temp = wrec(7)
+        pwind%mu = wrec(8)
+        cycle
+      end if
+      nwinds = nwinds + 1
+      wind_src(nwinds)%xc = wrec(1)
+      wind_src(nwinds)%yc = wrec(2)
+      wind_src(nwinds)%zc = wrec(3)
+      wind_src(nwinds)%vx = 0.0
+      wind_src(nwinds)%vy = 0.0
+      wind_src(nwinds)%vz = 0.0
+      wind_src(nwinds)%radius = wrec(4)
+      wind_src(nwinds)%mdot = wrec(5)
+      wind_src(nwinds)%vinf = wrec(6)
+      wind_src(nwinds)%temp = wrec(7)
+      wind_src(nwinds)%mu = wrec(8)
     end do
 
     allocate (buf(ncells_x, ncells_y, ncells_z, neqtot))
@@ -133,6 +149,7 @@ contains
     do n=1,nwinds
       call imposeSphericalWind (wind_src(n), uvars)
     end do
+    if (have_pwind) call imposePlaneWind (pwind, uvars)
 
   end subroutine userBoundary
 

@@ -640,6 +640,46 @@ static void cell_pos(const wxo_sim *s, int b, int i, int j, int k, double *x, do
   *z = ((zb - 1) * NC + (k - 1) + 0.5) * s->dx[2][l - 1];
 }
 
+/* imposePlaneWind (source/winds.f90:250-340): cells k = ncells_z..nzmax
+   (16..18), every i and j, of each block without a TOP neighbour
+   (neighborLevel gives none at the domain edge, amr.f90:439-500), whatever
+   the plane; the plane only picks the velocity's axis and sign
+   (winds.f90:300-313).  Hydro only: the reference leaves the field of the
+   state unset. */
+static void impose_plane_wind(wxo_sim *s, double *A) {
+  const wxo_config *c = &s->c;
+  int nv = c->nv;
+  const double dens = c->pw[0], vel = c->pw[1], temp = c->pw[2], mu = c->pw[3];
+  double vx = 0.0, vy = 0.0, vz = 0.0;
+  switch (c->pw_plane) {
+    case 1: vx = vel; break;
+    case 2: vx = -vel; break;
+    case 3: vy = vel; break;
+    case 4: vy = -vel; break;
+    case 5: vz = vel; break;
+    default: vz = -vel; break;
+  }
+  const double pres = dens / (mu * AMU) * KB * temp;
+  double pr[MAXV] = {0}, u[MAXV];
+  pr[0] = dens / c->d_sc;
+  pr[1] = vx / c->v_sc;
+  pr[2] = vy / c->v_sc;
+  pr[3] = vz / c->v_sc;
+  pr[4] = pres / c->p_sc;
+  wxo_prim2flow(c, pr, u);
+  for (int b = 0; b < s->nblk; b++) {
+    int xb, yb, zb, nx, ny, nz;
+    bcoords(s, s->bid[b], &xb, &yb, &zb);
+    nblev(s, s->lev[b], &nx, &ny, &nz);
+    if (zb != nz) continue;
+    double *Ab = A + (size_t)b * nv * NCELL;
+    for (int i = -1; i <= 18; i++)
+      for (int j = -1; j <= 18; j++)
+        for (int k = NC; k <= 18; k++)
+          for (int v = 0; v < nv; v++) Ab[v * NCELL + cid(i, j, k)] = u[v];
+  }
+}
+
 /* imposeSphericalWind (source/winds.f90:123-240), all cells of all blocks */
 static void impose_winds(wxo_sim *s, double *A) {
   const wxo_config *c = &s->c;
@@ -805,6 +845,7 @@ void wxo_boundary(wxo_sim *s, int depth, int which) {
     }
   }
   if (s->c.nwind > 0) impose_winds(s, A);
+  if (s->c.pw_plane > 0) impose_plane_wind(s, A);
 }
 
 /* ------------------------------------------------------------------ */

@@ -34,6 +34,8 @@ typedef struct {
   const double *tab_loglam;  /* ntab, log10(Lambda) */
   int nwind;
   const double *winds;       /* 8 per wind: xc,yc,zc,R,mdot,vinf,T,mu (cgs) */
+  int pw_plane;              /* plane wind: 0 none, PLANE_LEFT=1 .. PLANE_TOP=6 */
+  double pw[4];              /* its rho, vel, temp, mu (cgs) */
 } wxo_config;
 
 typedef struct wxo_sim wxo_sim;

@@ -51,6 +51,7 @@ TRAJ = {
     "snr_amr4": (5, [5]),
     "winds4": (5, [5]),
     "ot64_amr": (8, [2, 8]),
+    "plane_wind": (5, [1, 5]),
 }
 NSAMPLE = 4096
 # workloads run with admesh after every step: (steps, dumped steps)

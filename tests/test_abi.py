@@ -91,6 +91,16 @@ def test_download_checks_its_layout_arguments():
     assert L.wx_step(None, None, None, None, None, 1.0, 1.0) == -1
 
 
+def test_plane_wind_refuses_a_missing_context():
+    """wx_set_plane_wind checks its context before anything else (the
+    form and state checks need one, on a GPU)."""
+    if not os.path.exists(hip.LIBPATH):
+        pytest.skip("library not built")
+    L = hip.load()
+    assert L.wx_set_plane_wind(None, 6, 1e-23, 1e7, 1e4, 1.3) == -1
+    assert b"null context" in L.wx_last_error()
+
+
 def test_noalloc_eval_refuses_more_than_its_staging():
     """wx_eval_physics_noalloc stages through a fixed array: a call whose
     records exceed it is refused before any device call."""

@@ -36,7 +36,7 @@ def gpu(w, form, bids):
 
 # (workload, steps compared step by step against the oracle)
 PLAIN_CASES = [("sedov", 10), ("briowu", 30), ("ot64", 8), ("blast_reflect", 30), ("mhd_hlle", 20),
-               ("amr_small", 8), ("snr_amr4", 3), ("winds4", 3), ("ot64_amr", 8)]
+               ("amr_small", 8), ("snr_amr4", 3), ("winds4", 3), ("ot64_amr", 8), ("plane_wind", 5)]
 
 
 @pytest.mark.parametrize("name,nsteps", PLAIN_CASES)
@@ -133,7 +133,7 @@ def test_other_limiters_match_oracle(oracle, name, lim):
 
 # full reference trajectories: C1 Sedov 100 steps, C2 Brio-Wu to t = 0.1, ...
 @pytest.mark.parametrize("name", ["sedov", "briowu", "ot64", "blast_reflect", "mhd_hlle", "amr_small",
-                                  "snr_amr4", "winds4", "ot64_amr"])
+                                  "snr_amr4", "winds4", "ot64_amr", "plane_wind"])
 def test_plain_fp64_matches_reference_fixture(name):
     from walicxe3d_amd import hip
     fx = load_golden("traj_%s.npz" % name)
@@ -300,6 +300,21 @@ def test_fused_mhd_with_wind_sources(oracle, visc, solver):
     assert np.all(np.isfinite(ug))
     assert np.all(e <= 1e-5), e
     g.close()
+
+
+def test_plane_wind_needs_the_plain_hydro_form():
+    """A plane wind (imposePlaneWind) runs in the plain FP64 form for
+    hydro; the fused forms and MHD are refused with their reason, and the
+    context is released."""
+    from walicxe3d_amd import hip
+    w = W.get("plane_wind")
+    bids = np.asarray(w.blocks(), dtype=np.int32)
+    for form in (hip.FUSED_FP32, hip.FUSED_FP64):
+        with pytest.raises(hip.WxError, match="plain FP64 form"):
+            gpu(w, form, bids)
+    m = dataclasses.replace(W.get("mhd_winds4"), plane_wind=w.plane_wind)
+    with pytest.raises(hip.WxError, match="hydro only"):
+        gpu(m, hip.PLAIN_FP64, np.asarray(m.blocks(), dtype=np.int32))
 
 
 def drift(U0, u):

@@ -89,6 +89,7 @@ struct BidMap {
 struct Ctx {
   wx_config cfg{};
   std::vector<double> tabT, tabC, winds;
+  PlaneWind pw{};                    // wx_set_plane_wind (plain form); plane 0: none
   DevCfg dc{};
   int device = 0;
   hipStream_t st = nullptr;
@@ -133,6 +134,13 @@ void free_mesh(Ctx *c) {
   c->d_mark = nullptr;
   c->have_state = false;
   c->ghosts_recorded = false;
+}
+
+// the plane wind as the kernels take it (root blocks along z of the mesh)
+static PlaneWind plane_wind(const Ctx *c) {
+  PlaneWind pw = c->pw;
+  pw.nroot_z = c->nroot[2];
+  return pw;
 }
 
 // ---------------------------------------------------------------------------
@@ -480,12 +488,12 @@ int plain_timestep_raw(Ctx *c, double *dt) {
 
 int plain_godunov(Ctx *c, int order, double dt) {
   double dtp = order == 2 ? dt / 2.0 : dt;
-  plain_boundary(c->dc, c->dU, 1, c->st);
+  plain_boundary(c->dc, plane_wind(c), c->dU, 1, c->st);
   plain_prims(c->dc, c->dU, c->dP, WX_CELLS_GHOST, nullptr, c->st);
   plain_fluxes(c->dc, c->dP, c->dF, 1, c->st);
   plain_upwind(c->dc, c->dU, c->dP, c->dF, c->dUP, dtp, c->st);
   if (order == 2) {
-    plain_boundary(c->dc, c->dUP, 2, c->st);
+    plain_boundary(c->dc, plane_wind(c), c->dUP, 2, c->st);
     plain_prims(c->dc, c->dUP, c->dP, WX_CELLS_ALL, nullptr, c->st);
     plain_fluxes(c->dc, c->dP, c->dF, 2, c->st);
     plain_upwind(c->dc, c->dU, c->dP, c->dF, c->dUP, dt, c->st);
@@ -1158,7 +1166,7 @@ int wx_boundary(void *ctx, int depth, int which) {
   if (rc != WX_OK) return rc;
   if (is_fused(c)) return fail(WX_EINVAL, "plain-form entry point");
   if (depth != 1 && depth != 2) return fail(WX_EINVAL, "depth must be 1 or 2");
-  plain_boundary(c->dc, which == WX_UP ? c->dUP : c->dU, depth, c->st);
+  plain_boundary(c->dc, plane_wind(c), which == WX_UP ? c->dUP : c->dU, depth, c->st);
   return launch_check("boundary");
 }
 
@@ -1513,6 +1521,25 @@ int wx_write_vtk(void *ctx, const char *datadir, int noutput, int npassive, int 
     return fail(WX_EIO, err);
   std::snprintf(name, sizeof name, "Grid.%04d.vtk", noutput);
   if (vtk_grid_write((dir + name).c_str(), m, err)) return fail(WX_EIO, err);
+  return WX_OK;
+}
+
+int wx_set_plane_wind(void *ctx, int plane, double rho, double vel, double temp, double mu) {
+  Ctx *c = as_ctx(ctx);
+  if (!c) return fail(WX_EINVAL, "null context");
+  if (plane < 0 || plane > 6) return fail(WX_EINVAL, "plane must be 0 (none) or PLANE_LEFT=1 .. PLANE_TOP=6");
+  if (plane == 0) {
+    c->pw = PlaneWind{};
+    return WX_OK;
+  }
+  if (is_fused(c))
+    return fail(WX_EINVAL, "a plane wind needs the plain FP64 form (the fused forms impose spherical sources only)");
+  if (c->cfg.nv != 5)
+    return fail(WX_EINVAL, "a plane wind is hydro only (the reference leaves the field of its state unset)");
+  if (!(rho > 0.0) || !(temp > 0.0) || !(mu > 0.0) || !std::isfinite(rho) || !std::isfinite(vel) ||
+      !std::isfinite(temp) || !std::isfinite(mu))
+    return fail(WX_EINVAL, "plane wind: rho, temp and mu must be positive and finite, vel finite");
+  c->pw = PlaneWind{plane, 0, rho, vel, temp, mu};
   return WX_OK;
 }
 

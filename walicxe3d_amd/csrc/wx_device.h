@@ -173,6 +173,43 @@ __device__ inline bool wind_state(const DevCfg &c, int b, int i, int j, int k, d
   return hit;
 }
 
+// A plane wind (imposePlaneWind, source/winds.f90:250-340), imposed after the
+// spherical sources as a userBoundary calls them: cells k = 16..18 (every i,
+// j) of each block without a TOP neighbour, which neighborLevel reports at
+// the domain's top edge whatever the boundary type (source/amr.f90:439-523);
+// the plane only picks the velocity's axis and sign (winds.f90:300-313).
+// plane = 0: none.  Hydro only (the reference leaves the state's field
+// unset).  Plain FP64 form.
+struct PlaneWind {
+  int plane;        // PLANE_LEFT=1 .. PLANE_TOP=6 (source/winds.f90:93-98)
+  int nroot_z;      // root blocks along z
+  double rho, vel, temp, mu;   // g cm^-3, cm s^-1, K, mean mass per particle in AMU
+};
+
+__device__ inline bool plane_wind_state(const DevCfg &c, const PlaneWind &pw, int b, int k, double *u) {
+  if (pw.plane == 0 || k < NC) return false;
+  const BlockInfo &B = c.info[b];
+  if (B.coords[2] != (pw.nroot_z << (B.level - 1))) return false;
+  double vx = 0.0, vy = 0.0, vz = 0.0;
+  switch (pw.plane) {
+    case 1: vx = pw.vel; break;
+    case 2: vx = -pw.vel; break;
+    case 3: vy = pw.vel; break;
+    case 4: vy = -pw.vel; break;
+    case 5: vz = pw.vel; break;
+    default: vz = -pw.vel; break;
+  }
+  const double pres = pw.rho / (pw.mu * AMU) * KB * pw.temp;
+  double p[MAXV] = {0, 0, 0, 0, 0, 0, 0, 0};
+  p[0] = pw.rho / c.d_sc;
+  p[1] = vx / c.v_sc;
+  p[2] = vy / c.v_sc;
+  p[3] = vz / c.v_sc;
+  p[4] = pres / c.p_sc;
+  prim2flow(c.pd, p, u);
+  return true;
+}
+
 // ---------------------------------------------------------------------------
 // Cooling (apply_cooling, source/cooling.f90:285-371; find_coef :378-419;
 // calcTemp, source/hydro_core.f90:231-270).  Updates p and returns E.

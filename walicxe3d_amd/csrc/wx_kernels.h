@@ -8,7 +8,7 @@
 namespace wx {
 
 // plain FP64 form (wx_plain.hip)
-void plain_boundary(const DevCfg &c, double *A, int depth, hipStream_t st);
+void plain_boundary(const DevCfg &c, const PlaneWind &pw, double *A, int depth, hipStream_t st);
 void plain_prims(const DevCfg &c, const double *A, double *P, int cells, unsigned long long *bad, hipStream_t st);
 void plain_fluxes(const DevCfg &c, const double *P, double *F, int order, hipStream_t st);
 void plain_upwind(const DevCfg &c, const double *U, const double *P, const double *F, double *UP, double dtp,

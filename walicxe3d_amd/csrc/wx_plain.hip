@@ -109,7 +109,7 @@ __global__ void __launch_bounds__(256) k_boundary(DevCfg c, double *A, int depth
 
 // Wind sources over interior and face-ghost cells (userBoundary).
 // grid (28, nblk): 16 blocks of 256 interior cells, 12 of face ghosts.
-__global__ void __launch_bounds__(256) k_winds(DevCfg c, double *A) {
+__global__ void __launch_bounds__(256) k_winds(DevCfg c, double *A, PlaneWind pw) {
   int b = blockIdx.y;
   int i, j, k;
   if (blockIdx.x < 16) {
@@ -125,7 +125,9 @@ __global__ void __launch_bounds__(256) k_winds(DevCfg c, double *A) {
     i = d[0]; j = d[1]; k = d[2];
   }
   double u[MAXV];
-  if (wind_state(c, b, i, j, k, u)) {
+  bool hit = wind_state(c, b, i, j, k, u);
+  if (plane_wind_state(c, pw, b, k, u)) hit = true;   // after the spherical sources
+  if (hit) {
     int q = cid(i, j, k);
     for (int v = 0; v < c.nv; v++) A[((size_t)b * c.nv + v) * NCELL + q] = u[v];
   }
@@ -475,9 +477,9 @@ __global__ void __launch_bounds__(1024) k_mark_fused(DevCfg c, const double *U, 
 // ---------------------------------------------------------------------------
 // host launchers
 // ---------------------------------------------------------------------------
-void plain_boundary(const DevCfg &c, double *A, int depth, hipStream_t st) {
+void plain_boundary(const DevCfg &c, const PlaneWind &pw, double *A, int depth, hipStream_t st) {
   hipLaunchKernelGGL(k_boundary, dim3(6 * depth, c.nblk), dim3(256), 0, st, c, A, depth);
-  if (c.nwind > 0) hipLaunchKernelGGL(k_winds, dim3(28, c.nblk), dim3(256), 0, st, c, A);
+  if (c.nwind > 0 || pw.plane != 0) hipLaunchKernelGGL(k_winds, dim3(28, c.nblk), dim3(256), 0, st, c, A, pw);
 }
 void plain_prims(const DevCfg &c, const double *A, double *P, int cells, unsigned long long *bad,
                  hipStream_t st) {

@@ -29,7 +29,7 @@
 ! are not all zero stops with a message (zero passives stay zero under the
 ! reference's update, so they are left as they are).  userBoundary effects
 ! other than periodic faces (handled by the library) must be registered
-! through hydro_gpu_set_winds.
+! through hydro_gpu_set_winds (spherical sources) or hydro_gpu_set_plane_wind.
 !===============================================================================
 module hydro_solver
 
@@ -39,6 +39,7 @@ module hydro_solver
 
   private
   public :: hydroSolver, getTimestep, doViscousStep, hydro_gpu_set_winds, hydro_gpu_release
+  public :: hydro_gpu_set_plane_wind
   public :: hydro_gpu_set_form, hydro_gpu_state_changed
 
   type(c_ptr), save :: ctx = c_null_ptr
@@ -50,6 +51,8 @@ module hydro_solver
   real(c_double), allocatable, target, save :: tab_T(:), tab_C(:)
   real(c_double), allocatable, target, save :: wind_rec(:)
   integer, save :: nwind = 0
+  integer(c_int), save :: pw_plane = 0         ! plane wind (hydro_gpu_set_plane_wind)
+  real(c_double), save :: pw_state(4) = 0.0d0  ! rho, vel, temp, mu
 
 contains
 
@@ -64,6 +67,17 @@ contains
     if (n > 0) wind_rec(1:8*n) = reshape(rec, [8*n])
     nwind = n
   end subroutine hydro_gpu_set_winds
+
+  !> The plane wind of a userBoundary that calls imposePlaneWind
+  !! (source/winds.f90:250-340): plane (PLANE_LEFT=1 .. PLANE_TOP=6), rho,
+  !! vel, temp, mu in cgs.  Plain FP64 form and hydro only (the library
+  !! refuses it otherwise).  Call before the first hydroSolver.
+  subroutine hydro_gpu_set_plane_wind(plane, rho, vel, temp, mu)
+    integer, intent(in) :: plane
+    real(c_double), intent(in) :: rho, vel, temp, mu
+    pw_plane = plane
+    pw_state = [rho, vel, temp, mu]
+  end subroutine hydro_gpu_set_plane_wind
 
   subroutine hydro_gpu_release()
     integer(c_int) :: rc
@@ -194,6 +208,8 @@ contains
         cfg%ntab = 0
       end if
       call check(wx_init(cfg, ctx), "wx_init")
+      if (pw_plane /= 0) call check(wx_set_plane_wind(ctx, pw_plane, pw_state(1), pw_state(2), pw_state(3), &
+                                                      pw_state(4)), "wx_set_plane_wind")
       allocate(ubuf(nbMaxProc, neqhydro + neqmhd, nxmin:nxmax, nymin:nymax, nzmin:nzmax))
       allocate(lbuf(nbMaxProc))
     end if

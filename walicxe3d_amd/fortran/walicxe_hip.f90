@@ -133,6 +133,13 @@ module walicxe_hip
       type(c_ptr), value :: ctx
     end function
 
+    integer(c_int) function wx_set_plane_wind(ctx, plane, rho, vel, temp, mu) bind(C, name="wx_set_plane_wind")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: plane
+      real(c_double), value :: rho, vel, temp, mu
+    end function
+
     integer(c_int) function wx_admesh_plan(nroot, maxlev, nblk, bids, flags, flags_out, cap, nblk_new, bids_new) &
         bind(C, name="wx_admesh_plan")
       import :: c_int, c_ptr

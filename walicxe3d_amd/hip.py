@@ -69,6 +69,7 @@ SIGNATURES = {
     "wx_fluxes_update": (C.c_int, [C.c_void_p, C.c_int, C.c_double]),
     "wx_copy_raw": (C.c_int, [C.c_void_p, C.c_int, _dp]),
     "wx_set_tuning": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    "wx_set_plane_wind": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double]),
     "wx_last_timing": (C.c_int, [C.c_void_p, _dp, _dp, _dp]),
     "wx_time_stages": (C.c_int, [C.c_void_p, C.c_int]),
     "wx_remesh": (C.c_int, [C.c_void_p, C.c_int, _ip]),
@@ -238,6 +239,11 @@ class Walicxe3D:
         h = C.c_void_p()
         _check(L.wx_init(C.byref(self.cfg), C.byref(h)), "wx_init")
         self._h = h
+        if w.plane_wind:
+            # imposePlaneWind of the problem's userBoundary (plain FP64 form,
+            # hydro); on refusal the context is released by close()
+            plane, rho, vel, temp, mu = w.plane_wind
+            _check(L.wx_set_plane_wind(h, int(plane), rho, vel, temp, mu), "wx_set_plane_wind")
         self.set_mesh(bids if bids is not None else w.blocks())
         self.it = C.c_int(0)
         self.dt = C.c_double(0.0)

@@ -207,6 +207,9 @@ class Workload:
     refine_rule: Optional[str] = None
     refine_radii: Tuple[float, ...] = ()
     winds: Tuple[Tuple[float, ...], ...] = ()
+    # imposePlaneWind (source/winds.f90:250-340): (plane, rho, vel, temp, mu)
+    # in cgs, plane PLANE_LEFT=1 .. PLANE_TOP=6; () for none
+    plane_wind: Tuple[float, ...] = ()
     units: str = "code"          # units of the VTK output (units_type): "code" or "phys"
     oracle: bool = True
     description: str = ""
@@ -351,6 +354,13 @@ _reg(_w(name="mhd_winds4", mhd=True, solver="hlld", limiter="minmod", gamma_expr
         nroot=(1, 1, 1), maxlev=5, base_level=2, phys=(10.0 * PC, 10.0 * PC, 10.0 * PC), ic="winds", seed=5,
         nsteps=5, cooling=True, refine_rule="winds", oracle=False, **_PHYS_UNITS,
         description="three spherical wind sources on 4 AMR levels in a uniform field, cooling, MHD, HLLD"))
+# a plane wind entering through the top of the box (imposePlaneWind), 2
+# levels (one top block refined), HLLC, viscosity
+_reg(_w(name="plane_wind", mhd=False, solver="hllc", limiter="minmod", gamma_expr="5.0/3.0",
+        gamma=5.0 / 3.0, cfl=0.4, visc_eta=5.0e-3, eight_wave=False, bc=FREE6,
+        nroot=(1, 1, 1), maxlev=3, base_level=2, phys=(10.0 * PC, 10.0 * PC, 10.0 * PC), ic="ambient", seed=9,
+        nsteps=5, refine=(6,), plane_wind=(6.0, 10.0 * 1.3 * AMU, 1.0e7, 1.0e4, 1.3), **_PHYS_UNITS,
+        description="plane wind through the top face (10 cm^-3, 100 km/s) into a uniform medium, 2 levels, HLLC"))
 # C5 full size (GPU only): winds, root 8^3 at level 4, 5 levels (4..8), ~32k blocks; HLL and HLLC
 for _solver, _suffix in (("hll", ""), ("hllc", "_hllc")):
     _reg(_w(name="winds32k" + _suffix, mhd=False, solver=_solver, limiter="minmod", gamma_expr="5.0/3.0",
@@ -547,13 +557,13 @@ def block_ic(w: Workload, m: Mesh, bid: int) -> np.ndarray:
         bx = -np.sin(2.0 * pi * Y) * b0
         by = np.sin(4.0 * pi * X) * b0
         return _prim_to_cons(w, rho, vx, vy, vz, p, bx, by, zeros)
-    if w.ic in ("snr", "winds", "blast_amr"):
+    if w.ic in ("snr", "winds", "blast_amr", "ambient"):
         # ambient n_H = 1 cm^-3 at T = 1e4 K with lognormal clumps / +-5% noise
         # (+-0.01% for blast_amr)
         if w.ic == "snr":
             G = _smooth_gaussian(w, X, Y, Z)
             rho = np.exp(0.5 * G - 0.125)
-        elif w.ic == "winds":
+        elif w.ic in ("winds", "ambient"):
             rho = 1.0 + 0.05 * uniform_pm1(w.seed, 1, l, g)
         else:
             rho = 1.0 + 1.0e-4 * uniform_pm1(w.seed, 1, l, g)
@@ -619,6 +629,16 @@ def wind_params(w: Workload):
         mdot = rho_r * vinf * R * R * 4.0 * math.pi
         out.append((cen[0] * w.l_sc, cen[1] * w.l_sc, cen[2] * w.l_sc, R, mdot, vinf, 1.0e4, w.mu0))
     return out
+
+
+def plane_wind_record(w: Workload):
+    """The plane wind as a record of the reference harness's wind list:
+    (plane, 0, 0, -1, rho, vel, temp, mu), the negative radius marking it;
+    [] without one."""
+    if not w.plane_wind:
+        return []
+    plane, rho, vel, temp, mu = w.plane_wind
+    return [(float(plane), 0.0, 0.0, -1.0, rho, vel, temp, mu)]
 
 
 def initial_state(w: Workload):
