@@ -149,21 +149,23 @@ def _per_kernel(rows, counter):
     acc = collections.defaultdict(list)
     for r in rows:
         if r["Counter_Name"] == counter:
-            acc[r["Kernel_Name"]].append((int(r["Grid_Size"]), float(r["Counter_Value"])))
+            acc[r["Kernel_Name"]].append((int(r["Grid_Size"]), int(r["Dispatch_Id"]), float(r["Counter_Value"])))
     return acc
 
 
 def _copy_reading(acc):
-    """Counter value of the largest copy dispatch (the 1 GiB device copy)."""
+    """Counter value of the 1 GiB device copy that ends the run: the last of
+    the copy dispatches with the largest grid (the state upload is a copy of
+    the same grid, of another size)."""
     best = None
     for k, v in acc.items():
         if "copyBuffer" in k:
-            for grid, val in v:
-                if best is None or grid > best[0]:
-                    best = (grid, val)
+            for grid, disp, val in v:
+                if best is None or (grid, disp) > best[:2]:
+                    best = (grid, disp, val)
     if best is None:
         raise RuntimeError("copy kernel not found in the counter data")
-    return best[1]
+    return best[2]
 
 
 def _summarize(args):
@@ -190,8 +192,8 @@ def _summarize(args):
         st = _stage_of(k)
         if st is None:
             continue
-        f = sum(v for _, v in fetch[k]) / len(fetch[k])
-        wr = sum(v for _, v in write.get(k, [(0, 0.0)])) / max(1, len(write.get(k, [])))
+        f = sum(v for _, _, v in fetch[k]) / len(fetch[k])
+        wr = sum(v for _, _, v in write.get(k, [(0, 0, 0.0)])) / max(1, len(write.get(k, [])))
         hbm = 1024.0 * (f * fscale + wr * wscale)
         out[st] = (k, f, wr, hbm)
     for st in ("stage1", "stage2"):
