@@ -213,18 +213,18 @@ def test_emulated_mhd_with_wind_sources_match_the_oracle(emu, tmp_path, oracle):
     """The FP32 MHD stage kernels that carry the wind code: in a cell a
     source covers, each stage's update starts from the wind state (the
     reference's boundary() imposes it before each stage), viscous and
-    inviscid; within 1e-5 per variable of the oracle after 2 steps."""
+    inviscid; within 1e-5 per variable of the oracle after one step (without
+    the wind state the momenta are off by O(1) at once)."""
     import dataclasses
     from conftest import rel_l1
     for visc in (5.0e-3, 0.0):
         w = dataclasses.replace(W.get("mhd_winds4"), visc_eta=visc)
         p = str(tmp_path / ("mhd_winds_%g.bin" % visc))
         blocks, U0 = _write_input(p, w, hip.FUSED_FP32)
-        _run(emu, [p, "2"], str(tmp_path))
+        _run(emu, [p, "1"], str(tmp_path))
         o = oracle.Sim(w, blocks)
         o.set_state(U0)
-        for _ in range(2):
-            o.step()
+        o.step()
         uo, _ = o.get_state()
         e = rel_l1(np.fromfile(p + ".out").reshape(uo.shape), uo)
         assert np.all(e <= 1e-5), (visc, e)
