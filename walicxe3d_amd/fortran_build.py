@@ -1,8 +1,8 @@
 """Build of the Fortran ISO_C_BINDING layer (walicxe3d_amd/fortran) with
 amdflang: the binding module walicxe_hip and a layout probe.  The drop-in
 module hydro_solver_gpu.f90 depends on the host Walicxe3D build's own
-`parameters`/`globals` modules and is compiled inside that build (see
-INTEGRATION.md)."""
+`parameters`/`globals` modules and is compiled inside that build, in place
+of source/hydro_solver.f90."""
 import os
 import shutil
 import subprocess
