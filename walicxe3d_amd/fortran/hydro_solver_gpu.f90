@@ -69,14 +69,17 @@ contains
   end subroutine hydro_gpu_set_winds
 
   !> The plane wind of a userBoundary that calls imposePlaneWind
-  !! (source/winds.f90:250-340): plane (PLANE_LEFT=1 .. PLANE_TOP=6), rho,
-  !! vel, temp, mu in cgs.  Plain FP64 form and hydro only (the library
-  !! refuses it otherwise).  Call before the first hydroSolver.
+  !! (source/winds.f90:250-340): plane (PLANE_LEFT=1 .. PLANE_TOP=6; 0
+  !! removes it), rho, vel, temp, mu in cgs.  Plain FP64 form and hydro only
+  !! (the library refuses it otherwise).  Takes effect at the next
+  !! hydroSolver.
   subroutine hydro_gpu_set_plane_wind(plane, rho, vel, temp, mu)
     integer, intent(in) :: plane
     real(c_double), intent(in) :: rho, vel, temp, mu
     pw_plane = plane
     pw_state = [rho, vel, temp, mu]
+    if (c_associated(ctx)) call check(wx_set_plane_wind(ctx, pw_plane, pw_state(1), pw_state(2), &
+                                                        pw_state(3), pw_state(4)), "wx_set_plane_wind")
   end subroutine hydro_gpu_set_plane_wind
 
   subroutine hydro_gpu_release()
