@@ -52,6 +52,7 @@ TRAJ = {
     "winds4": (5, [5]),
     "ot64_amr": (8, [2, 8]),
     "plane_wind": (5, [1, 5]),
+    "sedov_hll1": (20, [20]),
 }
 NSAMPLE = 4096
 # workloads run with admesh after every step: (steps, dumped steps)

@@ -36,7 +36,8 @@ def gpu(w, form, bids):
 
 # (workload, steps compared step by step against the oracle)
 PLAIN_CASES = [("sedov", 10), ("briowu", 30), ("ot64", 8), ("blast_reflect", 30), ("mhd_hlle", 20),
-               ("amr_small", 8), ("snr_amr4", 3), ("winds4", 3), ("ot64_amr", 8), ("plane_wind", 5)]
+               ("amr_small", 8), ("snr_amr4", 3), ("winds4", 3), ("ot64_amr", 8), ("plane_wind", 5),
+               ("sedov_hll1", 20)]
 
 
 @pytest.mark.parametrize("name,nsteps", PLAIN_CASES)
@@ -133,7 +134,7 @@ def test_other_limiters_match_oracle(oracle, name, lim):
 
 # full reference trajectories: C1 Sedov 100 steps, C2 Brio-Wu to t = 0.1, ...
 @pytest.mark.parametrize("name", ["sedov", "briowu", "ot64", "blast_reflect", "mhd_hlle", "amr_small",
-                                  "snr_amr4", "winds4", "ot64_amr", "plane_wind"])
+                                  "snr_amr4", "winds4", "ot64_amr", "plane_wind", "sedov_hll1"])
 def test_plain_fp64_matches_reference_fixture(name):
     from walicxe3d_amd import hip
     fx = load_golden("traj_%s.npz" % name)

@@ -75,7 +75,7 @@ def _first_dump(fx):
 # (workload, compare at the first dump only: keeps the CPU suite short; the
 # GPU tests check the last dump)
 TRAJ = ["sedov", "briowu", "ot64", "blast_reflect", "mhd_hlle", "amr_small", "snr_amr4", "winds4", "ot64_amr",
-        "plane_wind"]
+        "plane_wind", "sedov_hll1"]
 
 
 @pytest.mark.parametrize("name", TRAJ)

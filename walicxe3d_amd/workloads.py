@@ -274,6 +274,11 @@ _reg(_w(name="sedov", mhd=False, solver="hllc", limiter="minmod", gamma_expr="5.
         gamma=5.0 / 3.0, cfl=0.4, visc_eta=5.0e-3, eight_wave=False, bc=FREE6,
         nroot=(1, 1, 1), maxlev=3, base_level=3, phys=(1.0, 1.0, 1.0), ic="sedov", seed=1, nsteps=100,
         description="3D Sedov-type blast, 64^3 cells as 4x4x4 blocks of 16^3, HLLC, minmod"))
+# the first-order scheme (SOLVER_HLL1: one Godunov(1) stage, hydro_solver.f90:89-90)
+_reg(_w(name="sedov_hll1", mhd=False, solver="hll1", limiter="minmod", gamma_expr="5.0/3.0",
+        gamma=5.0 / 3.0, cfl=0.4, visc_eta=5.0e-3, eight_wave=False, bc=FREE6,
+        nroot=(1, 1, 1), maxlev=3, base_level=3, phys=(1.0, 1.0, 1.0), ic="sedov", seed=1, nsteps=20,
+        description="the Sedov blast with the first-order HLL scheme (SOLVER_HLL1)"))
 # C1 at 256^3 (GPU throughput of the hydro path at the Orszag-Tang size)
 _reg(_w(name="sedov256", mhd=False, solver="hllc", limiter="minmod", gamma_expr="5.0/3.0",
         gamma=5.0 / 3.0, cfl=0.4, visc_eta=5.0e-3, eight_wave=False, bc=FREE6,
