@@ -53,6 +53,7 @@ TRAJ = {
     "ot64_amr": (8, [2, 8]),
     "plane_wind": (5, [1, 5]),
     "sedov_hll1": (20, [20]),
+    "amr_small_twotemp": (8, [2, 8]),
 }
 NSAMPLE = 4096
 # workloads run with admesh after every step: (steps, dumped steps)

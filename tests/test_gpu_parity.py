@@ -37,7 +37,7 @@ def gpu(w, form, bids):
 # (workload, steps compared step by step against the oracle)
 PLAIN_CASES = [("sedov", 10), ("briowu", 30), ("ot64", 8), ("blast_reflect", 30), ("mhd_hlle", 20),
                ("amr_small", 8), ("snr_amr4", 3), ("winds4", 3), ("ot64_amr", 8), ("plane_wind", 5),
-               ("sedov_hll1", 20)]
+               ("sedov_hll1", 20), ("amr_small_twotemp", 8)]
 
 
 @pytest.mark.parametrize("name,nsteps", PLAIN_CASES)
@@ -134,7 +134,8 @@ def test_other_limiters_match_oracle(oracle, name, lim):
 
 # full reference trajectories: C1 Sedov 100 steps, C2 Brio-Wu to t = 0.1, ...
 @pytest.mark.parametrize("name", ["sedov", "briowu", "ot64", "blast_reflect", "mhd_hlle", "amr_small",
-                                  "snr_amr4", "winds4", "ot64_amr", "plane_wind", "sedov_hll1"])
+                                  "snr_amr4", "winds4", "ot64_amr", "plane_wind", "sedov_hll1",
+                                  "amr_small_twotemp"])
 def test_plain_fp64_matches_reference_fixture(name):
     from walicxe3d_amd import hip
     fx = load_golden("traj_%s.npz" % name)
@@ -245,7 +246,8 @@ def test_plain_godunov_and_viscous_pieces(oracle):
 # keeps every O(1) flux part in FP64, the stated bar is 1e-5)
 FUSED_CASES = [("ot64", 20, 1e-7, 1e-5), ("sedov", 20, 1e-6, None), ("blast_reflect", 20, 1e-6, None),
                ("amr_small", 8, 1e-6, None), ("mhd_hlle", 10, 1e-7, 1e-5), ("winds4", 3, 1e-5, None),
-               ("snr_amr4", 3, 1e-6, None), ("ot64_amr", 10, 1e-7, 1e-5)]
+               ("snr_amr4", 3, 1e-6, None), ("ot64_amr", 10, 1e-7, 1e-5),
+               ("amr_small_twotemp", 8, 1e-6, None)]
 
 
 @pytest.mark.parametrize("name,nsteps,tol,tol_small", FUSED_CASES)

@@ -75,7 +75,7 @@ def _first_dump(fx):
 # (workload, compare at the first dump only: keeps the CPU suite short; the
 # GPU tests check the last dump)
 TRAJ = ["sedov", "briowu", "ot64", "blast_reflect", "mhd_hlle", "amr_small", "snr_amr4", "winds4", "ot64_amr",
-        "plane_wind", "sedov_hll1"]
+        "plane_wind", "sedov_hll1", "amr_small_twotemp"]
 
 
 @pytest.mark.parametrize("name", TRAJ)
@@ -98,3 +98,19 @@ def test_trajectory(oracle, name):
     u, p = sim.get_state()
     dev_sum, dev_s = compare_dump(w, fx, d, u, p, bids)
     assert dev_sum == 0.0 and dev_s == 0.0, (dev_sum, dev_s)
+
+
+def test_twotemp_fixture_tells_the_eos_apart(oracle):
+    """The two-temperature fixture pins calcTemp's ionised branch: the same
+    run with the adiabatic EOS (one mean molecular weight) misses it."""
+    import dataclasses
+    fx = load_golden("traj_amr_small_twotemp.npz")
+    w = dataclasses.replace(W.get("amr_small_twotemp"), eos="adiabatic")
+    bids = fx["bids"]
+    sim = oracle.Sim(w, bids)
+    sim.set_state(ic_in_order(w, bids))
+    for _ in range(_first_dump(fx)):
+        sim.step()
+    u, p = sim.get_state()
+    dev_sum, dev_s = compare_dump(w, fx, _first_dump(fx), u, p, bids)
+    assert dev_sum > 0.0 or dev_s > 0.0

@@ -324,6 +324,13 @@ _reg(_w(name="amr_small", mhd=False, solver="hllc", limiter="woodward", gamma_ex
         nroot=(1, 1, 1), maxlev=4, base_level=2, phys=(20.0 * PC, 20.0 * PC, 20.0 * PC), ic="snr", seed=8,
         nsteps=8, cooling=True, refine_rule="snr", units="phys", **_PHYS_UNITS,
         description="small SNR on 3 AMR levels with radiative cooling"))
+# the two-temperature equation of state (EOS_TWOTEMP: mui above ion_thres,
+# hydro_core.f90:250-256) in the cooling of the small SNR
+_reg(_w(name="amr_small_twotemp", mhd=False, solver="hllc", limiter="woodward", gamma_expr="5.0/3.0",
+        gamma=5.0 / 3.0, cfl=0.4, visc_eta=5.0e-3, eight_wave=False, bc=FREE6,
+        nroot=(1, 1, 1), maxlev=4, base_level=2, phys=(20.0 * PC, 20.0 * PC, 20.0 * PC), ic="snr", seed=8,
+        nsteps=8, cooling=True, refine_rule="snr", eos="twotemp", units="phys", **_PHYS_UNITS,
+        description="small SNR on 3 AMR levels with radiative cooling, two-temperature EOS"))
 # Mesh adaptation coverage (admesh every step): an off-centre blast in a
 # near-uniform medium, the initial refinement centred elsewhere, so that the
 # first updates refine around the blast, force coarser neighbours by the
