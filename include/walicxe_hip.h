@@ -103,6 +103,17 @@ int wx_set_mesh(void *ctx, const int nroot[3], int maxlev, const double *dxyz, i
    counterpart call in the reference: it replaces the user module's call. */
 int wx_set_plane_wind(void *ctx, int plane, double rho, double vel, double temp, double mu);
 
+/* A uniform acceleration (gx, gy, gz) in code units, the source term a
+   problem's get_user_source_terms (source/sources.f90:124, called per cell
+   from upwindStep, source/godunov.f90:266-271, with user_source_terms on)
+   adds from the cell's primitives: s(2:4) += rho*g, s(5) += rho*(v.g), ahead
+   of the Powell terms, so that each stage adds dtp*s.  The hook's arguments
+   (the cell's primitives and its in-block indices) carry no block position,
+   and this is the one such source the library runs.  on = 0 removes it.
+   Plain FP64 form (hydro and MHD): fused contexts get WX_EINVAL.  No
+   counterpart call in the reference: it replaces the user module's code. */
+int wx_set_uniform_gravity(void *ctx, int on, double gx, double gy, double gz);
+
 /* State transfer.  Upload sets U's interior, zeroes its ghosts and computes
    PRIM (updatePrims, source/hydro_core.f90:43-59).  For WX_LAYOUT_FORTRAN
    pass nbmax = nbMaxProc and local_blocks = localBlocks; slots whose bID is

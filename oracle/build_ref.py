@@ -95,6 +95,7 @@ def parameters_text(w, nprocs=1, amr=False):
         "SOLVER": SOLVER[w.solver], "LIMITER": LIMITER[w.limiter],
         "CFL": flit(w.cfl), "VISC": flit(w.visc_eta),
         "EIGHTWAVE": ".true." if w.eight_wave else ".false.",
+        "USERSRC": ".true." if w.gravity else ".false.",
         "COOLING": "COOL_TABLE" if w.cooling else "COOL_NONE",
         "COOLFILE": "cooltable.dat", "COOL_LIMIT": flit(w.cooling_limit),
         "GAMMA": w.gamma_expr, "MU0": flit(w.mu0), "MUI": flit(w.mui),
@@ -196,7 +197,7 @@ def main(argv):
     return 0
 
 
-DROPIN_WORKLOADS = ["sedov", "ot64", "amr_small", "blast_reflect", "plane_wind"]
+DROPIN_WORKLOADS = ["sedov", "ot64", "amr_small", "blast_reflect", "plane_wind", "sedov_gravity"]
 # (workload, MPI ranks): 16 ranks = the host cores one GPU of the box comes with
 BASELINE_BUILDS = [("ot64", 16)]
 # workloads whose fixtures include the reference's mesh adaptation

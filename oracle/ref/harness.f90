@@ -45,8 +45,8 @@ program wx_oracle_harness
   use hydro_core
   use hydro_solver, only : hydroSolver
 #ifdef WX_DROPIN
-  use hydro_solver, only : hydro_gpu_set_plane_wind
-  use userconds, only : have_pwind, pwind
+  use hydro_solver, only : hydro_gpu_set_plane_wind, hydro_gpu_set_gravity
+  use userconds, only : have_pwind, pwind, have_grav, grav
 #endif
   use coolingModule, only : cooling, find_coef
   use load_balance, only : loadBalance
@@ -86,9 +86,10 @@ program wx_oracle_harness
   call updatePrims ()
   if (nextout == 0) nextout = 1
 #ifdef WX_DROPIN
-  ! the GPU drop-in does not call userBoundary: a problem registers its
-  ! plane wind with it instead
+  ! the GPU drop-in does not call userBoundary or get_user_source_terms: a
+  ! problem registers its plane wind and its uniform acceleration with it
   if (have_pwind) call hydro_gpu_set_plane_wind (pwind%plane, pwind%rho, pwind%vel, pwind%temp, pwind%mu)
+  if (have_grav) call hydro_gpu_set_gravity (grav)
 #endif
 
   ! multi-rank runs (timing): the master writes the step log only

@@ -23,14 +23,18 @@
 ! Multi-rank runs (nProcs > 1): the root blocks start on the master
 ! (source/init.f90:766-786), so the master refines and fills every block and
 ! the harness's loadBalance then spreads them over the ranks.
-!   get_user_source_terms       - no user sources
+!   get_user_source_terms       - a uniform acceleration g when the file
+!                                 holds one (rho*g on the momenta, rho*(v.g)
+!                                 on the energy), else nothing
 !
 ! File layout (little endian, stream):
 !   int32 magic (0x43495857), nv, nrefine, nblocks, nwind
 !   int32 refine(nrefine)
 !   float64 wind(8, nwind)   xc,yc,zc,radius,mdot,vinf,temp,mu (cgs); a
-!                            record with radius < 0 is the plane wind:
-!                            plane,0,0,-1,rho,vel,temp,mu
+!                            record with radius -1 is the plane wind:
+!                            plane,0,0,-1,rho,vel,temp,mu; one with radius
+!                            -2 the uniform acceleration: gx,gy,gz,-2,0,0,0,0
+!                            (code units)
 !   nblocks x { int32 bID ; float64 u(16,16,16,nv) with i fastest, then j,k,
 !               then variable }
 !
@@ -49,6 +53,8 @@ module userconds
   type(spherical_wind_type) :: wind_src(maxwinds)
   logical :: have_pwind = .false.
   type(plane_wind_type) :: pwind
+  logical :: have_grav = .false.
+  real :: grav(3) = 0.0
 
 contains
 
@@ -90,6 +96,11 @@ contains
     nwinds = 0
     do n=1,nw
       read (77) wrec
+      if (wrec(4) < -1.5) then
+        have_grav = .true.
+        grav(1:3) = wrec(1:3)
+        cycle
+      end if
       if (wrec(4) < 0.0) then
         have_pwind = .true.
         pwind%plane = nint(wrec(1))
@@ -341,6 +352,12 @@ contains
     real, intent(in)    :: pp(neqtot)
     real, intent(inout) :: s(neqtot)
     integer, intent(in) :: i, j, k
+
+    if (.not. have_grav) return
+    s(2) = s(2) + pp(1)*grav(1)
+    s(3) = s(3) + pp(1)*grav(2)
+    s(4) = s(4) + pp(1)*grav(3)
+    s(5) = s(5) + pp(1)*(pp(2)*grav(1) + pp(3)*grav(2) + pp(4)*grav(3))
 
   end subroutine get_user_source_terms
 

@@ -940,21 +940,33 @@ static void block_update(wxo_sim *s, int b, int order, double dtp) {
           UPb[v * NCELL + q] = Ub[v * NCELL + q] + dtdx * (FX(0, v, i - 1, j, k) - FX(0, v, i, j, k))
                                + dtdy * (FX(1, v, i, j - 1, k) - FX(1, v, i, j, k))
                                + dtdz * (FX(2, v, i, j, k - 1) - FX(2, v, i, j, k));
-        if (c->eight_wave && nv == 8) {
-          /* source_function -> divbcorr_8w_source (sources.f90:62-133) */
+        if ((c->eight_wave && nv == 8) || c->user_src) {
+          /* source_function (sources.f90:100-133): s = 0, the user terms
+             (get_user_source_terms, :124), then divbcorr_8w_source (:62-95);
+             upwindStep adds dtp*s when eight_wave or user_source_terms
+             (godunov.f90:266-271) */
           double sv[MAXV] = {0};
-          double divB = (Pb[5 * NCELL + cid(i + 1, j, k)] - Pb[5 * NCELL + cid(i - 1, j, k)]) / (2.0 * s->dx[0][l - 1])
-                        + (Pb[6 * NCELL + cid(i, j + 1, k)] - Pb[6 * NCELL + cid(i, j - 1, k)]) / (2.0 * s->dx[1][l - 1])
-                        + (Pb[7 * NCELL + cid(i, j, k + 1)] - Pb[7 * NCELL + cid(i, j, k - 1)]) / (2.0 * s->dx[2][l - 1]);
           double pp[MAXV];
           gather(Pb, nv, q, pp);
-          sv[1] = sv[1] - divB * pp[5];
-          sv[2] = sv[2] - divB * pp[6];
-          sv[3] = sv[3] - divB * pp[7];
-          sv[4] = sv[4] - divB * (pp[1] * pp[5] + pp[2] * pp[6] + pp[3] * pp[7]);
-          sv[5] = sv[5] - divB * pp[1];
-          sv[6] = sv[6] - divB * pp[2];
-          sv[7] = sv[7] - divB * pp[3];
+          if (c->user_src) {
+            /* a uniform acceleration: rho*g on the momenta, rho*(v.g) on E */
+            sv[1] = sv[1] + pp[0] * c->grav[0];
+            sv[2] = sv[2] + pp[0] * c->grav[1];
+            sv[3] = sv[3] + pp[0] * c->grav[2];
+            sv[4] = sv[4] + pp[0] * (pp[1] * c->grav[0] + pp[2] * c->grav[1] + pp[3] * c->grav[2]);
+          }
+          if (c->eight_wave && nv == 8) {
+            double divB = (Pb[5 * NCELL + cid(i + 1, j, k)] - Pb[5 * NCELL + cid(i - 1, j, k)]) / (2.0 * s->dx[0][l - 1])
+                          + (Pb[6 * NCELL + cid(i, j + 1, k)] - Pb[6 * NCELL + cid(i, j - 1, k)]) / (2.0 * s->dx[1][l - 1])
+                          + (Pb[7 * NCELL + cid(i, j, k + 1)] - Pb[7 * NCELL + cid(i, j, k - 1)]) / (2.0 * s->dx[2][l - 1]);
+            sv[1] = sv[1] - divB * pp[5];
+            sv[2] = sv[2] - divB * pp[6];
+            sv[3] = sv[3] - divB * pp[7];
+            sv[4] = sv[4] - divB * (pp[1] * pp[5] + pp[2] * pp[6] + pp[3] * pp[7]);
+            sv[5] = sv[5] - divB * pp[1];
+            sv[6] = sv[6] - divB * pp[2];
+            sv[7] = sv[7] - divB * pp[3];
+          }
           for (int v = 0; v < nv; v++) UPb[v * NCELL + q] = UPb[v * NCELL + q] + dtp * sv[v];
         }
       }

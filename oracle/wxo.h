@@ -36,6 +36,8 @@ typedef struct {
   const double *winds;       /* 8 per wind: xc,yc,zc,R,mdot,vinf,T,mu (cgs) */
   int pw_plane;              /* plane wind: 0 none, PLANE_LEFT=1 .. PLANE_TOP=6 */
   double pw[4];              /* its rho, vel, temp, mu (cgs) */
+  int user_src;              /* user source term: a uniform acceleration */
+  double grav[3];            /* its gx, gy, gz (code units) */
 } wxo_config;
 
 typedef struct wxo_sim wxo_sim;

@@ -25,7 +25,8 @@ class Config(C.Structure):
                 ("ion_thres", C.c_double), ("d_sc", C.c_double), ("e_sc", C.c_double),
                 ("t_sc", C.c_double), ("p_sc", C.c_double), ("l_sc", C.c_double), ("v_sc", C.c_double),
                 ("ntab", C.c_int), ("tab_logT", _dp), ("tab_loglam", _dp),
-                ("nwind", C.c_int), ("winds", _dp), ("pw_plane", C.c_int), ("pw", C.c_double * 4)]
+                ("nwind", C.c_int), ("winds", _dp), ("pw_plane", C.c_int), ("pw", C.c_double * 4),
+                ("user_src", C.c_int), ("grav", C.c_double * 3)]
 
 
 _lib = None
@@ -109,6 +110,10 @@ def make_config(w, table=None, winds=None):
         c.pw_plane = int(w.plane_wind[0])
         for n in range(4):
             c.pw[n] = w.plane_wind[1 + n]
+    if w.gravity:
+        c.user_src = 1
+        for n in range(3):
+            c.grav[n] = w.gravity[n]
     c._keep = keep
     return c
 

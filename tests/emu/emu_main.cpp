@@ -51,6 +51,11 @@ static int run_case(const char *path, int nsteps) {
   int rc = wx_init(&cfg, &ctx); if (rc < 0) { printf("init %d %s\n", rc, wx_last_error()); return 1; }
   rc = wx_set_mesh(ctx, nroot, maxlev, dx.data(), nblk, bids.data()); if (rc < 0) { printf("mesh %d %s\n", rc, wx_last_error()); return 1; }
   rc = wx_upload_state(ctx, U.data(), WX_LAYOUT_BLOCKS, 0, nullptr); if (rc < 0) { printf("upload %d %s\n", rc, wx_last_error()); return 1; }
+  if (const char *g = getenv("EMU_GRAVITY")) {   // "gx gy gz": a uniform acceleration
+    double gx = 0, gy = 0, gz = 0;
+    if (sscanf(g, "%lf %lf %lf", &gx, &gy, &gz) != 3) { printf("EMU_GRAVITY needs three numbers\n"); return 1; }
+    rc = wx_set_uniform_gravity(ctx, 1, gx, gy, gz); if (rc < 0) { printf("gravity %d %s\n", rc, wx_last_error()); return 1; }
+  }
   if (getenv("EMU_NSLAB") && cfg.form != WX_FORM_PLAIN_FP64) {
     rc = wx_set_tuning(ctx, 4, atoi(getenv("EMU_NSLAB"))); if (rc < 0) { printf("slabs %d %s\n", rc, wx_last_error()); return 1; }
   }

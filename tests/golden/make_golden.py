@@ -54,6 +54,8 @@ TRAJ = {
     "plane_wind": (5, [1, 5]),
     "sedov_hll1": (20, [20]),
     "amr_small_twotemp": (8, [2, 8]),
+    "sedov_gravity": (20, [5, 20]),
+    "ot64_gravity": (8, [2, 8]),
 }
 NSAMPLE = 4096
 # workloads run with admesh after every step: (steps, dumped steps)

@@ -27,7 +27,7 @@ def write_ic_with_passive(path, w, blocks, U):
     nb = U.shape[0]
     U1 = np.concatenate([U, np.zeros((nb, 1) + U.shape[2:])], axis=1)
     refine = w.refine_order()
-    wp = W.wind_params(w) + W.plane_wind_record(w)
+    wp = W.wind_params(w) + W.plane_wind_record(w) + W.gravity_record(w)
     with open(path, "wb") as f:
         f.write(struct.pack("<5i", W.IC_MAGIC, w.nv + 1, len(refine), nb, len(wp)))
         if refine:

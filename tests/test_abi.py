@@ -101,6 +101,15 @@ def test_plane_wind_refuses_a_missing_context():
     assert b"null context" in L.wx_last_error()
 
 
+def test_uniform_gravity_refuses_a_missing_context():
+    """wx_set_uniform_gravity checks its context before anything else."""
+    if not os.path.exists(hip.LIBPATH):
+        pytest.skip("library not built")
+    L = hip.load()
+    assert L.wx_set_uniform_gravity(None, 1, 0.0, 0.0, -1.0) == -1
+    assert b"null context" in L.wx_last_error()
+
+
 def test_noalloc_eval_refuses_more_than_its_staging():
     """wx_eval_physics_noalloc stages through a fixed array: a call whose
     records exceed it is refused before any device call."""

@@ -230,6 +230,23 @@ def test_emulated_mhd_with_wind_sources_match_the_oracle(emu, tmp_path, oracle):
         assert np.all(e <= 1e-5), (visc, e)
 
 
+def test_emulated_uniform_gravity_reproduces_the_oracle(emu, tmp_path, oracle):
+    """The user source term (a uniform acceleration) in the plain FP64 form,
+    hydro and MHD (ahead of the Powell terms): bit-exact to the oracle after
+    two steps."""
+    for name in ("sedov_gravity", "ot64_gravity"):
+        w = W.get(name)
+        p = str(tmp_path / (name + ".bin"))
+        blocks, U0 = _write_input(p, w, hip.PLAIN_FP64)
+        _run(emu, [p, "2"], str(tmp_path), EMU_GRAVITY="%r %r %r" % tuple(w.gravity))
+        o = oracle.Sim(w, blocks)
+        o.set_state(U0)
+        o.step()
+        o.step()
+        uo, _ = o.get_state()
+        assert np.array_equal(np.fromfile(p + ".out").reshape(uo.shape), uo), name
+
+
 def test_emulated_admesh_reproduces_the_oracle(emu, tmp_path, oracle):
     """wx_admesh after every step (device flagging, host plan, remesh with
     PRIM carried) in the plain FP64 form, against the oracle's restatement of

@@ -19,7 +19,7 @@ from walicxe3d_amd import workloads as W
 pytestmark = pytest.mark.gpu
 sys.path.insert(0, GOLDEN)
 
-CASES = ["sedov", "ot64", "amr_small", "blast_reflect", "plane_wind"]
+CASES = ["sedov", "ot64", "amr_small", "blast_reflect", "plane_wind", "sedov_gravity"]
 
 
 @pytest.mark.parametrize("name", CASES)

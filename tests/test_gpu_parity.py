@@ -37,7 +37,7 @@ def gpu(w, form, bids):
 # (workload, steps compared step by step against the oracle)
 PLAIN_CASES = [("sedov", 10), ("briowu", 30), ("ot64", 8), ("blast_reflect", 30), ("mhd_hlle", 20),
                ("amr_small", 8), ("snr_amr4", 3), ("winds4", 3), ("ot64_amr", 8), ("plane_wind", 5),
-               ("sedov_hll1", 20), ("amr_small_twotemp", 8)]
+               ("sedov_hll1", 20), ("amr_small_twotemp", 8), ("sedov_gravity", 20), ("ot64_gravity", 8)]
 
 
 @pytest.mark.parametrize("name,nsteps", PLAIN_CASES)
@@ -135,7 +135,7 @@ def test_other_limiters_match_oracle(oracle, name, lim):
 # full reference trajectories: C1 Sedov 100 steps, C2 Brio-Wu to t = 0.1, ...
 @pytest.mark.parametrize("name", ["sedov", "briowu", "ot64", "blast_reflect", "mhd_hlle", "amr_small",
                                   "snr_amr4", "winds4", "ot64_amr", "plane_wind", "sedov_hll1",
-                                  "amr_small_twotemp"])
+                                  "amr_small_twotemp", "sedov_gravity", "ot64_gravity"])
 def test_plain_fp64_matches_reference_fixture(name):
     from walicxe3d_amd import hip
     fx = load_golden("traj_%s.npz" % name)
@@ -318,6 +318,18 @@ def test_plane_wind_needs_the_plain_hydro_form():
     m = dataclasses.replace(W.get("mhd_winds4"), plane_wind=w.plane_wind)
     with pytest.raises(hip.WxError, match="hydro only"):
         gpu(m, hip.PLAIN_FP64, np.asarray(m.blocks(), dtype=np.int32))
+
+
+def test_uniform_gravity_needs_the_plain_form():
+    """A user source term (the uniform acceleration) runs in the plain FP64
+    form; the fused forms are refused with their reason."""
+    from walicxe3d_amd import hip
+    for name in ("sedov_gravity", "ot64_gravity"):
+        w = W.get(name)
+        bids = np.asarray(w.blocks(), dtype=np.int32)
+        for form in (hip.FUSED_FP32, hip.FUSED_FP64):
+            with pytest.raises(hip.WxError, match="plain FP64 form"):
+                gpu(w, form, bids)
 
 
 def drift(U0, u):

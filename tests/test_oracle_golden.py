@@ -75,7 +75,7 @@ def _first_dump(fx):
 # (workload, compare at the first dump only: keeps the CPU suite short; the
 # GPU tests check the last dump)
 TRAJ = ["sedov", "briowu", "ot64", "blast_reflect", "mhd_hlle", "amr_small", "snr_amr4", "winds4", "ot64_amr",
-        "plane_wind", "sedov_hll1", "amr_small_twotemp"]
+        "plane_wind", "sedov_hll1", "amr_small_twotemp", "sedov_gravity", "ot64_gravity"]
 
 
 @pytest.mark.parametrize("name", TRAJ)
@@ -114,3 +114,24 @@ def test_twotemp_fixture_tells_the_eos_apart(oracle):
     u, p = sim.get_state()
     dev_sum, dev_s = compare_dump(w, fx, _first_dump(fx), u, p, bids)
     assert dev_sum > 0.0 or dev_s > 0.0
+
+
+@pytest.mark.parametrize("name", ["sedov_gravity", "ot64_gravity"])
+def test_gravity_fixture_pins_the_source_terms(oracle, name):
+    """The uniform-acceleration fixtures pin get_user_source_terms: without
+    it, or with its x and z components exchanged, the oracle misses them."""
+    import dataclasses
+    fx = load_golden("traj_%s.npz" % name)
+    w = W.get(name)
+    gx, gy, gz = w.gravity
+    bids = fx["bids"]
+    d = _first_dump(fx)
+    for g in ((), (gz, gy, gx)):
+        wm = dataclasses.replace(w, gravity=g)
+        sim = oracle.Sim(wm, bids)
+        sim.set_state(ic_in_order(wm, bids))
+        for _ in range(d):
+            sim.step()
+        u, p = sim.get_state()
+        dev_sum, dev_s = compare_dump(wm, fx, d, u, p, bids)
+        assert dev_sum > 0.0 or dev_s > 0.0, g

@@ -90,6 +90,7 @@ struct Ctx {
   wx_config cfg{};
   std::vector<double> tabT, tabC, winds;
   PlaneWind pw{};                    // wx_set_plane_wind (plain form); plane 0: none
+  UserSource us{};                   // wx_set_uniform_gravity (plain form); on 0: none
   DevCfg dc{};
   int device = 0;
   hipStream_t st = nullptr;
@@ -491,12 +492,12 @@ int plain_godunov(Ctx *c, int order, double dt) {
   plain_boundary(c->dc, plane_wind(c), c->dU, 1, c->st);
   plain_prims(c->dc, c->dU, c->dP, WX_CELLS_GHOST, nullptr, c->st);
   plain_fluxes(c->dc, c->dP, c->dF, 1, c->st);
-  plain_upwind(c->dc, c->dU, c->dP, c->dF, c->dUP, dtp, c->st);
+  plain_upwind(c->dc, c->us, c->dU, c->dP, c->dF, c->dUP, dtp, c->st);
   if (order == 2) {
     plain_boundary(c->dc, plane_wind(c), c->dUP, 2, c->st);
     plain_prims(c->dc, c->dUP, c->dP, WX_CELLS_ALL, nullptr, c->st);
     plain_fluxes(c->dc, c->dP, c->dF, 2, c->st);
-    plain_upwind(c->dc, c->dU, c->dP, c->dF, c->dUP, dt, c->st);
+    plain_upwind(c->dc, c->us, c->dU, c->dP, c->dF, c->dUP, dt, c->st);
   }
   return launch_check("godunov");
 }
@@ -1186,7 +1187,7 @@ int wx_fluxes_update(void *ctx, int order, double dtp) {
   if (rc != WX_OK) return rc;
   if (is_fused(c)) return fail(WX_EINVAL, "plain-form entry point");
   plain_fluxes(c->dc, c->dP, c->dF, order, c->st);
-  plain_upwind(c->dc, c->dU, c->dP, c->dF, c->dUP, dtp, c->st);
+  plain_upwind(c->dc, c->us, c->dU, c->dP, c->dF, c->dUP, dtp, c->st);
   return launch_check("fluxes_update");
 }
 
@@ -1540,6 +1541,22 @@ int wx_set_plane_wind(void *ctx, int plane, double rho, double vel, double temp,
       !std::isfinite(temp) || !std::isfinite(mu))
     return fail(WX_EINVAL, "plane wind: rho, temp and mu must be positive and finite, vel finite");
   c->pw = PlaneWind{plane, 0, rho, vel, temp, mu};
+  return WX_OK;
+}
+
+int wx_set_uniform_gravity(void *ctx, int on, double gx, double gy, double gz) {
+  Ctx *c = as_ctx(ctx);
+  if (!c) return fail(WX_EINVAL, "null context");
+  if (on == 0) {
+    c->us = UserSource{};
+    return WX_OK;
+  }
+  if (on != 1) return fail(WX_EINVAL, "on must be 0 (none) or 1");
+  if (is_fused(c))
+    return fail(WX_EINVAL, "a user source term needs the plain FP64 form (the fused forms carry none)");
+  if (!std::isfinite(gx) || !std::isfinite(gy) || !std::isfinite(gz))
+    return fail(WX_EINVAL, "uniform gravity: gx, gy and gz must be finite");
+  c->us = UserSource{1, {gx, gy, gz}};
   return WX_OK;
 }
 

@@ -210,6 +210,24 @@ __device__ inline bool plane_wind_state(const DevCfg &c, const PlaneWind &pw, in
   return true;
 }
 
+// A uniform acceleration g (code units) as a problem's get_user_source_terms
+// (called per cell by source_function, source/sources.f90:100-133, in
+// upwindStep, source/godunov.f90:266-271) would add it: s(2:4) += rho*g,
+// s(5) += rho*(v.g), before the Powell terms.  The hook receives a cell's
+// primitives and in-block indices only, so a source of that interface cannot
+// depend on the block's position.  on = 0: none.  Plain FP64 form.
+struct UserSource {
+  int on;
+  double g[3];
+};
+
+__device__ inline void user_source(const UserSource &us, const double *pp, double *s) {
+  s[1] = s[1] + pp[0] * us.g[0];
+  s[2] = s[2] + pp[0] * us.g[1];
+  s[3] = s[3] + pp[0] * us.g[2];
+  s[4] = s[4] + pp[0] * (pp[1] * us.g[0] + pp[2] * us.g[1] + pp[3] * us.g[2]);
+}
+
 // ---------------------------------------------------------------------------
 // Cooling (apply_cooling, source/cooling.f90:285-371; find_coef :378-419;
 // calcTemp, source/hydro_core.f90:231-270).  Updates p and returns E.

@@ -11,8 +11,8 @@ namespace wx {
 void plain_boundary(const DevCfg &c, const PlaneWind &pw, double *A, int depth, hipStream_t st);
 void plain_prims(const DevCfg &c, const double *A, double *P, int cells, unsigned long long *bad, hipStream_t st);
 void plain_fluxes(const DevCfg &c, const double *P, double *F, int order, hipStream_t st);
-void plain_upwind(const DevCfg &c, const double *U, const double *P, const double *F, double *UP, double dtp,
-                  hipStream_t st);
+void plain_upwind(const DevCfg &c, const UserSource &us, const double *U, const double *P, const double *F,
+                  double *UP, double dtp, hipStream_t st);
 void plain_viscous(const DevCfg &c, const double *UP, double *U, hipStream_t st);
 void plain_timestep(const DevCfg &c, const double *P, unsigned long long *dtbits, hipStream_t st);
 void plain_cooling(const DevCfg &c, double *U, double *P, double dt, hipStream_t st);

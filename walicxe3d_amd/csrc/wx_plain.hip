@@ -218,10 +218,11 @@ __device__ inline int fidx(int s, int a1, int a2) { return ((a2 - 1) * NC + (a1 
 
 // ---------------------------------------------------------------------------
 // upwindStep (source/godunov.f90:199-277) with source_function
-// (source/sources.f90:100-133, Powell terms :38-95).  grid (16, nblk).
+// (source/sources.f90:100-133: the user terms, then Powell's :38-95).
+// grid (16, nblk).
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_upwind(DevCfg c, const double *U, const double *P, const double *F,
-                                                double *UP, double dtp) {
+                                                double *UP, double dtp, UserSource us) {
   int b = blockIdx.y;
   int t = blockIdx.x * 256 + threadIdx.x;
   int i = t % NC + 1, j = (t / NC) % NC + 1, k = t / (NC * NC) + 1;
@@ -244,21 +245,25 @@ __global__ void __launch_bounds__(256) k_upwind(DevCfg c, const double *U, const
     upv[v] = Ub[v * NCELL + q] + dtdx * (Fx[o + ix0] - Fx[o + ix1]) + dtdy * (Fy[o + iy0] - Fy[o + iy1]) +
              dtdz * (Fz[o + iz0] - Fz[o + iz1]);
   }
-  if (c.eight_wave && nv == 8) {
+  const bool powell = c.eight_wave && nv == 8;
+  if (powell || us.on) {
     const double *Pb = P + (size_t)b * nv * NCELL;
-    double divB = (Pb[5 * NCELL + q + 1] - Pb[5 * NCELL + q - 1]) / (2.0 * dx) +
-                  (Pb[6 * NCELL + q + NB] - Pb[6 * NCELL + q - NB]) / (2.0 * dy) +
-                  (Pb[7 * NCELL + q + NB * NB] - Pb[7 * NCELL + q - NB * NB]) / (2.0 * dz);
     double pp[MAXV];
     for (int v = 0; v < nv; v++) pp[v] = Pb[v * NCELL + q];
     double s[MAXV] = {0, 0, 0, 0, 0, 0, 0, 0};
-    s[1] = s[1] - divB * pp[5];
-    s[2] = s[2] - divB * pp[6];
-    s[3] = s[3] - divB * pp[7];
-    s[4] = s[4] - divB * (pp[1] * pp[5] + pp[2] * pp[6] + pp[3] * pp[7]);
-    s[5] = s[5] - divB * pp[1];
-    s[6] = s[6] - divB * pp[2];
-    s[7] = s[7] - divB * pp[3];
+    if (us.on) user_source(us, pp, s);
+    if (powell) {
+      double divB = (Pb[5 * NCELL + q + 1] - Pb[5 * NCELL + q - 1]) / (2.0 * dx) +
+                    (Pb[6 * NCELL + q + NB] - Pb[6 * NCELL + q - NB]) / (2.0 * dy) +
+                    (Pb[7 * NCELL + q + NB * NB] - Pb[7 * NCELL + q - NB * NB]) / (2.0 * dz);
+      s[1] = s[1] - divB * pp[5];
+      s[2] = s[2] - divB * pp[6];
+      s[3] = s[3] - divB * pp[7];
+      s[4] = s[4] - divB * (pp[1] * pp[5] + pp[2] * pp[6] + pp[3] * pp[7]);
+      s[5] = s[5] - divB * pp[1];
+      s[6] = s[6] - divB * pp[2];
+      s[7] = s[7] - divB * pp[3];
+    }
     for (int v = 0; v < nv; v++) upv[v] = upv[v] + dtp * s[v];
   }
   for (int v = 0; v < nv; v++) UPb[v * NCELL + q] = upv[v];
@@ -488,9 +493,9 @@ void plain_prims(const DevCfg &c, const double *A, double *P, int cells, unsigne
 void plain_fluxes(const DevCfg &c, const double *P, double *F, int order, hipStream_t st) {
   hipLaunchKernelGGL(k_fluxes, dim3(3 * 17, c.nblk), dim3(256), 0, st, c, P, F, order);
 }
-void plain_upwind(const DevCfg &c, const double *U, const double *P, const double *F, double *UP, double dtp,
-                  hipStream_t st) {
-  hipLaunchKernelGGL(k_upwind, dim3(16, c.nblk), dim3(256), 0, st, c, U, P, F, UP, dtp);
+void plain_upwind(const DevCfg &c, const UserSource &us, const double *U, const double *P, const double *F,
+                  double *UP, double dtp, hipStream_t st) {
+  hipLaunchKernelGGL(k_upwind, dim3(16, c.nblk), dim3(256), 0, st, c, U, P, F, UP, dtp, us);
 }
 void plain_viscous(const DevCfg &c, const double *UP, double *U, hipStream_t st) {
   hipLaunchKernelGGL(k_viscous, dim3(16, c.nblk), dim3(256), 0, st, c, UP, U);

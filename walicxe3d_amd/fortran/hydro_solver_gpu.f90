@@ -30,6 +30,9 @@
 ! reference's update, so they are left as they are).  userBoundary effects
 ! other than periodic faces (handled by the library) must be registered
 ! through hydro_gpu_set_winds (spherical sources) or hydro_gpu_set_plane_wind.
+! With user_source_terms on, get_user_source_terms runs on the device only as
+! a uniform acceleration registered through hydro_gpu_set_gravity (plain
+! form); without one the run stops with a message.
 !===============================================================================
 module hydro_solver
 
@@ -39,7 +42,7 @@ module hydro_solver
 
   private
   public :: hydroSolver, getTimestep, doViscousStep, hydro_gpu_set_winds, hydro_gpu_release
-  public :: hydro_gpu_set_plane_wind
+  public :: hydro_gpu_set_plane_wind, hydro_gpu_set_gravity
   public :: hydro_gpu_set_form, hydro_gpu_state_changed
 
   type(c_ptr), save :: ctx = c_null_ptr
@@ -53,6 +56,8 @@ module hydro_solver
   integer, save :: nwind = 0
   integer(c_int), save :: pw_plane = 0         ! plane wind (hydro_gpu_set_plane_wind)
   real(c_double), save :: pw_state(4) = 0.0d0  ! rho, vel, temp, mu
+  integer(c_int), save :: grav_on = 0          ! uniform acceleration (hydro_gpu_set_gravity)
+  real(c_double), save :: grav_g(3) = 0.0d0    ! gx, gy, gz (code units)
 
 contains
 
@@ -81,6 +86,18 @@ contains
     if (c_associated(ctx)) call check(wx_set_plane_wind(ctx, pw_plane, pw_state(1), pw_state(2), &
                                                         pw_state(3), pw_state(4)), "wx_set_plane_wind")
   end subroutine hydro_gpu_set_plane_wind
+
+  !> The source term of a get_user_source_terms (source/sources.f90:124)
+  !! that adds a uniform acceleration g (code units): rho*g to the momenta,
+  !! rho*(v.g) to the energy.  Plain FP64 form (the library refuses it
+  !! otherwise).  Takes effect at the next hydroSolver.
+  subroutine hydro_gpu_set_gravity(g)
+    real(c_double), intent(in) :: g(3)
+    grav_on = 1
+    grav_g = g
+    if (c_associated(ctx)) call check(wx_set_uniform_gravity(ctx, grav_on, grav_g(1), grav_g(2), grav_g(3)), &
+                                      "wx_set_uniform_gravity")
+  end subroutine hydro_gpu_set_gravity
 
   subroutine hydro_gpu_release()
     integer(c_int) :: rc
@@ -213,6 +230,8 @@ contains
       call check(wx_init(cfg, ctx), "wx_init")
       if (pw_plane /= 0) call check(wx_set_plane_wind(ctx, pw_plane, pw_state(1), pw_state(2), pw_state(3), &
                                                       pw_state(4)), "wx_set_plane_wind")
+      if (grav_on /= 0) call check(wx_set_uniform_gravity(ctx, grav_on, grav_g(1), grav_g(2), grav_g(3)), &
+                                   "wx_set_uniform_gravity")
       allocate(ubuf(nbMaxProc, neqhydro + neqmhd, nxmin:nxmax, nymin:nymax, nzmin:nzmax))
       allocate(lbuf(nbMaxProc))
     end if
@@ -298,10 +317,11 @@ contains
       write(logu,'(a)') "SOLVER_LAX does not run on the GPU"
       call clean_abort (ERROR_GENERIC)
     end if
-    if (user_source_terms) then
+    if (user_source_terms .and. grav_on == 0) then
       ! get_user_source_terms (source/sources.f90:124) is problem code the
-      ! device update does not call
-      write(logu,'(a)') "user_source_terms: get_user_source_terms does not run on the GPU"
+      ! device update does not call; a uniform acceleration is registered
+      ! through hydro_gpu_set_gravity
+      write(logu,'(a)') "user_source_terms: get_user_source_terms does not run on the GPU (see hydro_gpu_set_gravity)"
       call clean_abort (ERROR_GENERIC)
     end if
     call check_passives()

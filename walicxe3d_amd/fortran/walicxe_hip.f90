@@ -140,6 +140,13 @@ module walicxe_hip
       real(c_double), value :: rho, vel, temp, mu
     end function
 
+    integer(c_int) function wx_set_uniform_gravity(ctx, on, gx, gy, gz) bind(C, name="wx_set_uniform_gravity")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: on
+      real(c_double), value :: gx, gy, gz
+    end function
+
     integer(c_int) function wx_admesh_plan(nroot, maxlev, nblk, bids, flags, flags_out, cap, nblk_new, bids_new) &
         bind(C, name="wx_admesh_plan")
       import :: c_int, c_ptr

@@ -70,6 +70,7 @@ SIGNATURES = {
     "wx_copy_raw": (C.c_int, [C.c_void_p, C.c_int, _dp]),
     "wx_set_tuning": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "wx_set_plane_wind": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double]),
+    "wx_set_uniform_gravity": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_double]),
     "wx_last_timing": (C.c_int, [C.c_void_p, _dp, _dp, _dp]),
     "wx_time_stages": (C.c_int, [C.c_void_p, C.c_int]),
     "wx_remesh": (C.c_int, [C.c_void_p, C.c_int, _ip]),
@@ -244,6 +245,10 @@ class Walicxe3D:
             # hydro); on refusal the context is released by close()
             plane, rho, vel, temp, mu = w.plane_wind
             _check(L.wx_set_plane_wind(h, int(plane), rho, vel, temp, mu), "wx_set_plane_wind")
+        if w.gravity:
+            # the problem's get_user_source_terms (plain FP64 form)
+            gx, gy, gz = w.gravity
+            _check(L.wx_set_uniform_gravity(h, 1, gx, gy, gz), "wx_set_uniform_gravity")
         self.set_mesh(bids if bids is not None else w.blocks())
         self.it = C.c_int(0)
         self.dt = C.c_double(0.0)

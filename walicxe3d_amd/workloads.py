@@ -210,6 +210,9 @@ class Workload:
     # imposePlaneWind (source/winds.f90:250-340): (plane, rho, vel, temp, mu)
     # in cgs, plane PLANE_LEFT=1 .. PLANE_TOP=6; () for none
     plane_wind: Tuple[float, ...] = ()
+    # a uniform acceleration (gx, gy, gz) in code units, the problem's
+    # get_user_source_terms (source/sources.f90:124); () for none
+    gravity: Tuple[float, ...] = ()
     units: str = "code"          # units of the VTK output (units_type): "code" or "phys"
     oracle: bool = True
     description: str = ""
@@ -274,6 +277,14 @@ _reg(_w(name="sedov", mhd=False, solver="hllc", limiter="minmod", gamma_expr="5.
         gamma=5.0 / 3.0, cfl=0.4, visc_eta=5.0e-3, eight_wave=False, bc=FREE6,
         nroot=(1, 1, 1), maxlev=3, base_level=3, phys=(1.0, 1.0, 1.0), ic="sedov", seed=1, nsteps=100,
         description="3D Sedov-type blast, 64^3 cells as 4x4x4 blocks of 16^3, HLLC, minmod"))
+# a user source term (get_user_source_terms, source/sources.f90:124;
+# user_source_terms on): the blast under a uniform acceleration whose three
+# components differ
+_reg(_w(name="sedov_gravity", mhd=False, solver="hllc", limiter="minmod", gamma_expr="5.0/3.0",
+        gamma=5.0 / 3.0, cfl=0.4, visc_eta=5.0e-3, eight_wave=False, bc=FREE6,
+        nroot=(1, 1, 1), maxlev=3, base_level=3, phys=(1.0, 1.0, 1.0), ic="sedov", seed=1, nsteps=20,
+        gravity=(3.0, -5.0, -9.0),
+        description="Sedov blast 64^3 under a uniform acceleration (user source terms)"))
 # the first-order scheme (SOLVER_HLL1: one Godunov(1) stage, hydro_solver.f90:89-90)
 _reg(_w(name="sedov_hll1", mhd=False, solver="hll1", limiter="minmod", gamma_expr="5.0/3.0",
         gamma=5.0 / 3.0, cfl=0.4, visc_eta=5.0e-3, eight_wave=False, bc=FREE6,
@@ -295,6 +306,13 @@ _reg(_w(name="ot64", mhd=True, solver="hlld", limiter="minmod", gamma_expr="5.0/
         gamma=5.0 / 3.0, cfl=0.4, visc_eta=0.0, eight_wave=True, bc=PER6,
         nroot=(1, 1, 1), maxlev=3, base_level=3, phys=(1.0, 1.0, 1.0), ic="ot", seed=3, nsteps=20,
         description="Orszag-Tang vortex 64^3 (reduced C3), periodic, HLLD"))
+# the user source ahead of the Powell terms (source_function,
+# source/sources.f90:121-129): Orszag-Tang under a uniform acceleration
+_reg(_w(name="ot64_gravity", mhd=True, solver="hlld", limiter="minmod", gamma_expr="5.0/3.0",
+        gamma=5.0 / 3.0, cfl=0.4, visc_eta=0.0, eight_wave=True, bc=PER6,
+        nroot=(1, 1, 1), maxlev=3, base_level=3, phys=(1.0, 1.0, 1.0), ic="ot", seed=3, nsteps=8,
+        gravity=(0.5, -1.0, -2.0),
+        description="Orszag-Tang vortex 64^3 under a uniform acceleration (user source terms), HLLD"))
 # MHD across coarse-fine faces: Orszag-Tang on two levels (level-2 blocks 2
 # and 5 stay coarse, the other six are refined), free boundaries (a periodic
 # face must wrap onto a block of its own level)
@@ -651,6 +669,16 @@ def plane_wind_record(w: Workload):
         return []
     plane, rho, vel, temp, mu = w.plane_wind
     return [(float(plane), 0.0, 0.0, -1.0, rho, vel, temp, mu)]
+
+
+def gravity_record(w: Workload):
+    """The uniform acceleration as a record of the reference harness's wind
+    list: (gx, gy, gz, -2, 0, 0, 0, 0), the radius -2 marking it; [] without
+    one."""
+    if not w.gravity:
+        return []
+    gx, gy, gz = w.gravity
+    return [(gx, gy, gz, -2.0, 0.0, 0.0, 0.0, 0.0)]
 
 
 def initial_state(w: Workload):
