@@ -53,7 +53,8 @@ extern "C" {
    run-time struct.  Enumerations use the reference's values
    (source/constants.f90:52-104). */
 typedef struct wx_config {
-  int nv;            /* neqtot: 5 hydro, 8 MHD (npassive = 0) */
+  int nv;            /* neqtot: 5 hydro, 8 MHD; 6 or 7 hydro with one or two
+                        passive scalars (plain FP64 form, no wind sources) */
   int mhd;           /* parameters `mhd` */
   int solver;        /* SOLVER_HLL1=2, HLL=3, HLLC=4, HLLE=5, HLLD=6 */
   int limiter;       /* LIMITER_NO_AVERAGE=-1 ... SUPERBEE=6 */

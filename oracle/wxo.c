@@ -59,6 +59,10 @@ struct wxo_sim {
 /* hydro_core.f90                                                      */
 /* ------------------------------------------------------------------ */
 
+/* flow variables ahead of the passive scalars (neq): 8 with a field, else 5;
+   variables nflow..nv-1 are passive scalars (hydro: nv = 5 + npassive) */
+static int nflow(const wxo_config *c) { return c->nv == 8 ? 8 : 5; }
+
 /* flow2prim (source/hydro_core.f90:323-378); returns istat */
 int wxo_flow2prim(const wxo_config *c, const double *u, double *p) {
   int istat = 0;
@@ -72,6 +76,7 @@ int wxo_flow2prim(const wxo_config *c, const double *u, double *p) {
   if (p[4] < 1.0e-30) { p[4] = 1.0e-30; istat = 1; }
   if (p[0] < 1.0e-40) { p[0] = 1.0e-40; istat = 2; }
   if (c->nv == 8) { p[5] = u[5]; p[6] = u[6]; p[7] = u[7]; }
+  for (int q = nflow(c); q < c->nv; q++) p[q] = u[q];   /* passives (:372-374) */
   return istat;
 }
 
@@ -87,6 +92,7 @@ void wxo_prim2flow(const wxo_config *c, const double *p, double *u) {
     if (c->mhd) u[4] = u[4] + 0.5 * (p[5] * p[5] + p[6] * p[6] + p[7] * p[7]);
     u[5] = p[5]; u[6] = p[6]; u[7] = p[7];
   }
+  for (int q = nflow(c); q < c->nv; q++) u[q] = p[q];   /* passives (:307-309) */
 }
 
 /* swapxy / swapxz (source/hydro_core.f90:448-496) */
@@ -122,6 +128,7 @@ void wxo_prim2fluxes(const wxo_config *c, const double *pin, int dim, double *f)
              - p[5] * (p[1] * p[5] + p[2] * p[6] + p[3] * p[7]);
     }
   }
+  for (int q = nflow(c); q < nv; q++) f[q] = p[1] * p[q];   /* passives (:429-431) */
   if (dim == 2) swapxy(nv, f); else if (dim == 3) swapxz(nv, f);
 }
 
@@ -265,7 +272,8 @@ static void primfhllc(const wxo_config *c, const double *pL, const double *pR, d
   ust[2] = rhost * pK[2];
   ust[3] = rhost * pK[3];
   ust[4] = rhost * (EK / pK[0] + (sst - pK[1]) * (sst + pK[4] / (pK[0] * (sK - pK[1]))));
-  for (int q = 5; q < nv; q++) ust[q] = 0.0;  /* HD only: nv == 5 */
+  /* passives (hllc.f90:262-264, 283-285); HLLC is hydro only */
+  for (int q = 5; q < nv; q++) ust[q] = rhost * pK[q] / pK[0];
   for (int q = 0; q < nv; q++) ff[q] = fK[q] + sK * (ust[q] - uK[q]);
 }
 

@@ -80,6 +80,7 @@ def make_config(w, table=None, winds=None):
     """Config struct for a workload; keeps the numpy buffers alive on it."""
     from walicxe3d_amd import workloads as W
     c = Config()
+    assert w.mhd or w.nv < 8, "hydro carries at most two passive scalars"
     c.nv = w.nv
     c.mhd = 1 if w.mhd else 0
     c.solver = W.SOLVERS[w.solver]

@@ -56,6 +56,9 @@ TRAJ = {
     "amr_small_twotemp": (8, [2, 8]),
     "sedov_gravity": (20, [5, 20]),
     "ot64_gravity": (8, [2, 8]),
+    "sedov_passive": (20, [5, 20]),
+    "blast_reflect_passive": (30, [10, 30]),
+    "amr_small_passive": (8, [2, 8]),
 }
 NSAMPLE = 4096
 # workloads run with admesh after every step: (steps, dumped steps)

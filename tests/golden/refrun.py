@@ -22,14 +22,25 @@ def harness_path(name: str) -> str:
     return os.path.join(REFDIR, name, "harness")
 
 
+def neq_ref(w):
+    """neqtot of the reference builds: the flow variables and one passive
+    scalar (a workload's own tracer, or zero everywhere)."""
+    assert w.npassive <= 1, "the reference builds carry one passive scalar"
+    return (8 if w.mhd else 5) + 1
+
+
 def write_ic_with_passive(path, w, blocks, U):
-    """The oracle builds carry one passive scalar (zero everywhere)."""
+    """The oracle builds carry one passive scalar: the workload's tracer, or
+    zero everywhere."""
     nb = U.shape[0]
-    U1 = np.concatenate([U, np.zeros((nb, 1) + U.shape[2:])], axis=1)
+    if w.npassive:
+        U1 = U
+    else:
+        U1 = np.concatenate([U, np.zeros((nb, 1) + U.shape[2:])], axis=1)
     refine = w.refine_order()
     wp = W.wind_params(w) + W.plane_wind_record(w) + W.gravity_record(w)
     with open(path, "wb") as f:
-        f.write(struct.pack("<5i", W.IC_MAGIC, w.nv + 1, len(refine), nb, len(wp)))
+        f.write(struct.pack("<5i", W.IC_MAGIC, neq_ref(w), len(refine), nb, len(wp)))
         if refine:
             f.write(np.asarray(refine, dtype="<i4").tobytes())
         for rec in wp:
@@ -131,8 +142,8 @@ def run(w, nsteps, dumps, workdir=None, unit_op=None, unit_in=None, timeout=3600
             blocks_after[s] = read_ints(os.path.join(wd, "blocks_%05d.bin" % s))
     for d in dumps:
         nb = len(blocks_after[d]) if (amr and d > 0) else nblk
-        u = read_state(os.path.join(wd, "u_%05d.bin" % d), nb, w.nv + 1)[:, : w.nv]
-        p = read_state(os.path.join(wd, "prim_%05d.bin" % d), nb, w.nv + 1)[:, : w.nv]
+        u = read_state(os.path.join(wd, "u_%05d.bin" % d), nb, neq_ref(w))[:, : w.nv]
+        p = read_state(os.path.join(wd, "prim_%05d.bin" % d), nb, neq_ref(w))[:, : w.nv]
         states[d] = (u.copy(), p.copy())
     out = dict(topo=topo, steps=steps, states=states, ic_blocks=blocks, ic_U=U, dumps=list(dumps))
     if bindump:

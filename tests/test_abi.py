@@ -49,10 +49,18 @@ def test_rejects_bad_config_before_touching_a_device():
     from walicxe3d_amd import workloads as W
     L = hip.load()
     cfg = hip.make_config(W.get("sedov"))
-    cfg.nv = 6
+    cfg.nv = 9
     h = C.c_void_p()
     assert L.wx_init(C.byref(cfg), C.byref(h)) == -1
     assert b"nv" in L.wx_last_error()
+    # passive scalars (nv = 6, 7): plain form, hydro, no wind sources
+    cfg = hip.make_config(W.get("sedov_passive"), form=hip.FUSED_FP32)
+    assert L.wx_init(C.byref(cfg), C.byref(h)) == -1
+    assert b"plain FP64 form" in L.wx_last_error()
+    cfg = hip.make_config(W.get("sedov_passive"))
+    cfg.mhd = 1
+    assert L.wx_init(C.byref(cfg), C.byref(h)) == -1
+    assert b"hydro" in L.wx_last_error()
     cfg = hip.make_config(W.get("sedov"))
     cfg.solver = 6   # HLLD needs MHD
     assert L.wx_init(C.byref(cfg), C.byref(h)) == -1
@@ -60,8 +68,7 @@ def test_rejects_bad_config_before_touching_a_device():
     cfg.solver = 2   # first-order HLL runs only in the plain form
     assert L.wx_init(C.byref(cfg), C.byref(h)) == -1
     assert L.wx_set_mesh(None, None, 1, None, 1, None) == -1
-    # EOS_H_RATE reads the neutral fraction from a passive scalar, which the
-    # library does not carry
+    # EOS_H_RATE (the neutral fraction from a passive scalar) is not taken
     cfg = hip.make_config(W.get("amr_small"))
     cfg.eos = 3
     assert L.wx_init(C.byref(cfg), C.byref(h)) == -1

@@ -247,6 +247,26 @@ def test_emulated_uniform_gravity_reproduces_the_oracle(emu, tmp_path, oracle):
         assert np.array_equal(np.fromfile(p + ".out").reshape(uo.shape), uo), name
 
 
+def test_emulated_passive_scalars_reproduce_the_oracle(emu, tmp_path, oracle):
+    """Passive scalars in the plain FP64 form (HLLC, HLL with reflecting
+    walls, HLLC on AMR levels with cooling; one tracer, and two on the
+    blast): bit-exact to the oracle, tracers included."""
+    import dataclasses
+    cases = [(W.get("sedov_passive"), 2), (dataclasses.replace(W.get("sedov_passive"), npassive=2), 2),
+             (W.get("blast_reflect_passive"), 2), (W.get("amr_small_passive"), 1)]
+    for n, (w, steps) in enumerate(cases):
+        p = str(tmp_path / ("passive%d.bin" % n))
+        blocks, U0 = _write_input(p, w, hip.PLAIN_FP64)
+        _run(emu, [p, str(steps)], str(tmp_path))
+        o = oracle.Sim(w, blocks)
+        o.set_state(U0)
+        for _ in range(steps):
+            o.step()
+        uo, _ = o.get_state()
+        u = np.fromfile(p + ".out").reshape(uo.shape)
+        assert np.array_equal(u, uo), (w.name, w.npassive, [float(np.abs(u[:, v] - uo[:, v]).max()) for v in range(w.nv)])
+
+
 def test_emulated_admesh_reproduces_the_oracle(emu, tmp_path, oracle):
     """wx_admesh after every step (device flagging, host plan, remesh with
     PRIM carried) in the plain FP64 form, against the oracle's restatement of

@@ -37,7 +37,8 @@ def gpu(w, form, bids):
 # (workload, steps compared step by step against the oracle)
 PLAIN_CASES = [("sedov", 10), ("briowu", 30), ("ot64", 8), ("blast_reflect", 30), ("mhd_hlle", 20),
                ("amr_small", 8), ("snr_amr4", 3), ("winds4", 3), ("ot64_amr", 8), ("plane_wind", 5),
-               ("sedov_hll1", 20), ("amr_small_twotemp", 8), ("sedov_gravity", 20), ("ot64_gravity", 8)]
+               ("sedov_hll1", 20), ("amr_small_twotemp", 8), ("sedov_gravity", 20), ("ot64_gravity", 8),
+               ("sedov_passive", 20), ("blast_reflect_passive", 30), ("amr_small_passive", 8)]
 
 
 @pytest.mark.parametrize("name,nsteps", PLAIN_CASES)
@@ -135,7 +136,8 @@ def test_other_limiters_match_oracle(oracle, name, lim):
 # full reference trajectories: C1 Sedov 100 steps, C2 Brio-Wu to t = 0.1, ...
 @pytest.mark.parametrize("name", ["sedov", "briowu", "ot64", "blast_reflect", "mhd_hlle", "amr_small",
                                   "snr_amr4", "winds4", "ot64_amr", "plane_wind", "sedov_hll1",
-                                  "amr_small_twotemp", "sedov_gravity", "ot64_gravity"])
+                                  "amr_small_twotemp", "sedov_gravity", "ot64_gravity", "sedov_passive",
+                                  "blast_reflect_passive", "amr_small_passive"])
 def test_plain_fp64_matches_reference_fixture(name):
     from walicxe3d_amd import hip
     fx = load_golden("traj_%s.npz" % name)
@@ -330,6 +332,21 @@ def test_uniform_gravity_needs_the_plain_form():
         for form in (hip.FUSED_FP32, hip.FUSED_FP64):
             with pytest.raises(hip.WxError, match="plain FP64 form"):
                 gpu(w, form, bids)
+
+
+def test_passive_scalars_need_the_plain_hydro_form():
+    """Passive scalars run in the plain FP64 form for hydro; the fused forms
+    and wind sources (whose state leaves the reference's passives unset) are
+    refused with their reason."""
+    from walicxe3d_amd import hip
+    w = W.get("sedov_passive")
+    bids = np.asarray(w.blocks(), dtype=np.int32)
+    for form in (hip.FUSED_FP32, hip.FUSED_FP64):
+        with pytest.raises(hip.WxError, match="plain FP64 form"):
+            gpu(w, form, bids)
+    ww = dataclasses.replace(W.get("winds4"), npassive=1)
+    with pytest.raises(hip.WxError, match="passives of the wind state"):
+        gpu(ww, hip.PLAIN_FP64, np.asarray(ww.blocks(), dtype=np.int32))
 
 
 def drift(U0, u):

@@ -75,7 +75,8 @@ def _first_dump(fx):
 # (workload, compare at the first dump only: keeps the CPU suite short; the
 # GPU tests check the last dump)
 TRAJ = ["sedov", "briowu", "ot64", "blast_reflect", "mhd_hlle", "amr_small", "snr_amr4", "winds4", "ot64_amr",
-        "plane_wind", "sedov_hll1", "amr_small_twotemp", "sedov_gravity", "ot64_gravity"]
+        "plane_wind", "sedov_hll1", "amr_small_twotemp", "sedov_gravity", "ot64_gravity",
+        "sedov_passive", "blast_reflect_passive", "amr_small_passive"]
 
 
 @pytest.mark.parametrize("name", TRAJ)
@@ -135,3 +136,26 @@ def test_gravity_fixture_pins_the_source_terms(oracle, name):
         u, p = sim.get_state()
         dev_sum, dev_s = compare_dump(wm, fx, d, u, p, bids)
         assert dev_sum > 0.0 or dev_s > 0.0, g
+
+
+@pytest.mark.parametrize("name,steps,tol", [("blast_reflect_passive", 10, 0.0), ("sedov_passive", 5, 1e-14)])
+def test_a_tracer_equal_to_the_density_stays_the_density(oracle, name, steps, tol):
+    """A passive scalar advected as a density (hydro_core.f90:307,372,429)
+    that starts equal to the density stays equal to it: under HLL its flux
+    is the mass flux term for term, so bit for bit; under HLLC its star
+    state rho* rho_K / rho_K is rho* to rounding (hllc.f90:262-264,
+    283-285).  A passive flux that drops the star state or takes the wrong
+    velocity fails."""
+    w = W.get(name)
+    bids = np.asarray(w.blocks(), dtype=np.int32)
+    U0 = ic_in_order(w, bids)
+    U0[:, 5] = U0[:, 0]
+    sim = oracle.Sim(w, bids)
+    sim.set_state(U0)
+    for _ in range(steps):
+        sim.step()
+    u, p = sim.get_state()
+    assert not np.array_equal(u[:, 0], U0[:, 0])
+    d = np.abs(u[:, 5] - u[:, 0]).max() / np.abs(u[:, 0]).max()
+    assert d <= tol, d
+    assert np.array_equal(p[:, 5], u[:, 5])

@@ -268,8 +268,11 @@ int build_topology(Ctx *c, const int *bids) {
 void fill_devcfg(Ctx *c, const double *dxyz) {
   DevCfg &d = c->dc;
   const wx_config &g = c->cfg;
-  d.pd = Phys<double>{g.nv, g.mhd, g.solver, g.limiter, g.gamma, g.cv};
-  d.pf = Phys<float>{g.nv, g.mhd, g.solver, g.limiter, (float)g.gamma, (float)g.cv};
+  // the physics takes the flow variables (neq); d.nv counts the stored ones
+  // (neqtot), passive scalars included
+  const int neq = g.nv == 8 ? 8 : 5;
+  d.pd = Phys<double>{neq, g.mhd, g.solver, g.limiter, g.gamma, g.cv};
+  d.pf = Phys<float>{neq, g.mhd, g.solver, g.limiter, (float)g.gamma, (float)g.cv};
   d.nv = g.nv;
   d.eight_wave = g.eight_wave;
   d.cooling = g.cooling;
@@ -823,8 +826,17 @@ const char *wx_version(void) { return "walicxe3d-amd 0.1 (gfx950)"; }
 
 int wx_init(const wx_config *cfg, void **out) {
   if (!cfg || !out) return fail(WX_EINVAL, "null argument");
-  if (cfg->nv != 5 && cfg->nv != 8) return fail(WX_EINVAL, "nv must be 5 or 8");
+  if (cfg->nv < 5 || cfg->nv > 8) return fail(WX_EINVAL, "nv must be 5 or 8, or 6..7 (hydro with passive scalars)");
   if (cfg->solver < 2 || cfg->solver > 6) return fail(WX_EINVAL, "unsupported solver");
+  // passive scalars (neqtot = 5 + npassive, hydro_core.f90:307,372,429,
+  // hllc.f90:262-285): the plain form's hydro kernels carry one or two
+  if (cfg->nv == 6 || cfg->nv == 7) {
+    if (cfg->form != WX_FORM_PLAIN_FP64)
+      return fail(WX_EINVAL, "passive scalars need the plain FP64 form (the fused forms carry the flow variables)");
+    if (cfg->mhd) return fail(WX_EINVAL, "passive scalars are carried for hydro (nv = 5 + npassive) only");
+    if (cfg->nwind)
+      return fail(WX_EINVAL, "wind sources with passive scalars: the reference leaves the passives of the wind state unset");
+  }
   if ((cfg->solver == 5 || cfg->solver == 6) && cfg->nv != 8) return fail(WX_EINVAL, "HLLE/HLLD need nv = 8");
   if (cfg->limiter < -1 || cfg->limiter > 6) return fail(WX_EINVAL, "unknown limiter");
   if (cfg->cooling && (cfg->ntab < 2 || cfg->ntab > MAXTAB || !cfg->tab_logT || !cfg->tab_loglam))
@@ -841,8 +853,8 @@ int wx_init(const wx_config *cfg, void **out) {
     return fail(WX_EINVAL, "the fused form with nv = 8 runs HLLE or HLLD");
   if (cfg->form != WX_FORM_PLAIN_FP64 && cfg->nv == 8 && !cfg->mhd)
     return fail(WX_EINVAL, "the fused form with nv = 8 needs mhd = 1");
-  // calcTemp (source/hydro_core.f90:231-270): EOS_H_RATE (3) takes the
-  // neutral fraction from a passive scalar, which the library does not carry
+  // calcTemp (source/hydro_core.f90:231-270): EOS_H_RATE (3), which takes the
+  // neutral fraction from the first passive scalar, is not implemented
   if (cfg->eos < 0 || cfg->eos > 2) return fail(WX_EINVAL, "eos must be EOS_ADIABATIC, EOS_SINGLE_SPECIE or EOS_TWOTEMP");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(WX_EHIP, "no HIP device");
@@ -1429,9 +1441,18 @@ int wx_hilbert_keys(const int nroot[3], int maxlev, int nblk, const int *bids, i
   return WX_OK;
 }
 
+// The writers take the flow variables of the state and write npassive zero
+// passive columns: a context that carries its own passive scalars is refused.
+static int writers_take_flow_variables(const Ctx *c) {
+  if (c->cfg.nv == 6 || c->cfg.nv == 7)
+    return fail(WX_EINVAL, "the output writers take contexts without passive scalars");
+  return WX_OK;
+}
+
 int wx_write_bin(void *ctx, const char *path, int npassive) {
   Ctx *c = as_ctx(ctx);
   int rc = need_state(c);
+  if (rc == WX_OK) rc = writers_take_flow_variables(c);
   if (rc != WX_OK) return rc;
   if (!path || npassive < 0) return fail(WX_EINVAL, "bad argument");
   std::vector<double> u((size_t)c->nblk * c->cfg.nv * NINT);
@@ -1472,6 +1493,7 @@ int wx_read_state(const char *path, double t_sc, double *time, int *it, int *nex
 int wx_write_vtk(void *ctx, const char *datadir, int noutput, int npassive, int units) {
   Ctx *c = as_ctx(ctx);
   int rc = need_state(c);
+  if (rc == WX_OK) rc = writers_take_flow_variables(c);
   if (rc != WX_OK) return rc;
   if (!datadir || npassive < 0 || noutput < 0 || noutput > 9999 || (units != 1 && units != 2))
     return fail(WX_EINVAL, "bad argument");
@@ -1535,6 +1557,8 @@ int wx_set_plane_wind(void *ctx, int plane, double rho, double vel, double temp,
   }
   if (is_fused(c))
     return fail(WX_EINVAL, "a plane wind needs the plain FP64 form (the fused forms impose spherical sources only)");
+  if (c->cfg.nv == 6 || c->cfg.nv == 7)
+    return fail(WX_EINVAL, "a plane wind with passive scalars: the reference leaves the passives of the wind state unset");
   if (c->cfg.nv != 5)
     return fail(WX_EINVAL, "a plane wind is hydro only (the reference leaves the field of its state unset)");
   if (!(rho > 0.0) || !(temp > 0.0) || !(mu > 0.0) || !std::isfinite(rho) || !std::isfinite(vel) ||

@@ -164,9 +164,62 @@ __global__ void __launch_bounds__(256) k_prims(DevCfg c, const double *A, double
   if (ghost && Ab[q] == 0.0) return;
   for (int v = 0; v < nv; v++) u[v] = Ab[v * NCELL + q];
   int st = flow2prim(c.pd, u, p);
+  for (int v = c.pd.nv; v < nv; v++) p[v] = u[v];   // passive scalars (hydro_core.f90:372-374)
   double *Pb = P + (size_t)b * nv * NCELL;
   for (int v = 0; v < nv; v++) Pb[v * NCELL + q] = p[v];
   if (st && bad && !ghost) atomicAdd(bad, 1ull);
+}
+
+// ---------------------------------------------------------------------------
+// Passive scalars of the hydro solvers, variables pd.nv..nv-1, on states
+// already turned to the x direction: a passive's physical flux is u*s
+// (prim2fluxes, hydro_core.f90:429-431) and its conserved value s
+// (prim2flow :307-309).  HLL and HLL1 take the HLL average
+// (hll.f90:186-212); HLLC the star state rho* s_K / rho_K of the side the
+// contact leaves (hllc.f90:232-296: 262-264, 283-285).  The wave speeds
+// repeat those of the solver, with the same arithmetic.
+// ---------------------------------------------------------------------------
+__device__ inline void passive_fluxes(const Phys<double> &c, int nv, const double *pL, const double *pR,
+                                      double *ff) {
+  const double cl = sound(c, pL), cr = sound(c, pR);
+  if (c.solver == SOLVER_HLLC) {
+    const double dl1 = pL[0], dr1 = pR[0], ul1 = pL[1], ur1 = pR[1], pl1 = pL[4], pr1 = pR[4];
+    const double sl = rfmin(ul1 - cl, ur1 - cr);
+    const double sr = rfmax(ul1 + cl, ur1 + cr);
+    const double sst =
+        (pr1 - pl1 + dl1 * ul1 * (sl - ul1) - dr1 * ur1 * (sr - ur1)) / (dl1 * (sl - ul1) - dr1 * (sr - ur1));
+    if (sl > 0.0) {
+      for (int q = c.nv; q < nv; q++) ff[q] = pL[1] * pL[q];
+      return;
+    }
+    if (sr < 0.0) {
+      for (int q = c.nv; q < nv; q++) ff[q] = pR[1] * pR[q];
+      return;
+    }
+    const double *pK;
+    double sK;
+    if (sst >= 0.0) { pK = pL; sK = sl; }
+    else if (sst <= 0.0) { pK = pR; sK = sr; }
+    else {
+      for (int q = c.nv; q < nv; q++) ff[q] = NAN;
+      return;
+    }
+    const double rhost = pK[0] * (sK - pK[1]) / (sK - sst);
+    for (int q = c.nv; q < nv; q++) ff[q] = pK[1] * pK[q] + sK * (rhost * pK[q] / pK[0] - pK[q]);
+    return;
+  }
+  const double sl = rfmin(pL[1] - cl, pR[1] - cr);
+  const double sr = rfmax(pL[1] + cl, pR[1] + cr);
+  if (sl >= 0.0) {
+    for (int q = c.nv; q < nv; q++) ff[q] = pL[1] * pL[q];
+    return;
+  }
+  if (sr <= 0.0) {
+    for (int q = c.nv; q < nv; q++) ff[q] = pR[1] * pR[q];
+    return;
+  }
+  for (int q = c.nv; q < nv; q++)
+    ff[q] = (sr * (pL[1] * pL[q]) - sl * (pR[1] * pR[q]) + sl * sr * (pR[q] - pL[q])) / (sr - sl);
 }
 
 // ---------------------------------------------------------------------------
@@ -207,6 +260,7 @@ __global__ void __launch_bounds__(256) k_fluxes(DevCfg c, const double *P, doubl
     for (int v = 0; v < nv; v++) limit1(pll[v], pl[v], pr[v], prr[v], c.pd.limiter);
   }
   riemann(c.pd, pl, pr, ff);
+  if (nv > c.pd.nv) passive_fluxes(c.pd, nv, pl, pr, ff);
   swapdim(nv, dim, ff);
   // interface index: position s fastest along dim, then t1, then t2
   int idx = ((a2 - 1) * NC + (a1 - 1)) * 17 + s;

@@ -213,13 +213,17 @@ class Workload:
     # a uniform acceleration (gx, gy, gz) in code units, the problem's
     # get_user_source_terms (source/sources.f90:124); () for none
     gravity: Tuple[float, ...] = ()
+    # passive scalars (npassive, advected as densities: hydro_core.f90:307,
+    # 372, 429; hllc.f90:262-285); hydro, plain form
+    npassive: int = 0
     units: str = "code"          # units of the VTK output (units_type): "code" or "phys"
     oracle: bool = True
     description: str = ""
 
     @property
     def nv(self) -> int:
-        return 8 if self.mhd else 5
+        """neqtot: the flow variables and the passive scalars."""
+        return (8 if self.mhd else 5) + self.npassive
 
     @property
     def cv(self) -> float:
@@ -285,6 +289,21 @@ _reg(_w(name="sedov_gravity", mhd=False, solver="hllc", limiter="minmod", gamma_
         nroot=(1, 1, 1), maxlev=3, base_level=3, phys=(1.0, 1.0, 1.0), ic="sedov", seed=1, nsteps=20,
         gravity=(3.0, -5.0, -9.0),
         description="Sedov blast 64^3 under a uniform acceleration (user source terms)"))
+# passive scalars (one tracer, advected as a density), hydro: HLLC on the
+# blast, HLL with reflecting walls, HLLC on three AMR levels with cooling
+_reg(_w(name="sedov_passive", mhd=False, solver="hllc", limiter="minmod", gamma_expr="5.0/3.0",
+        gamma=5.0 / 3.0, cfl=0.4, visc_eta=5.0e-3, eight_wave=False, bc=FREE6,
+        nroot=(1, 1, 1), maxlev=3, base_level=3, phys=(1.0, 1.0, 1.0), ic="sedov", seed=1, nsteps=20,
+        npassive=1, description="Sedov blast 64^3 carrying a passive tracer, HLLC"))
+_reg(_w(name="blast_reflect_passive", mhd=False, solver="hll", limiter="vanleer", gamma_expr="1.4",
+        gamma=1.4, cfl=0.4, visc_eta=5.0e-3, eight_wave=False, bc=("reflective",) * 6,
+        nroot=(1, 1, 1), maxlev=2, base_level=2, phys=(1.0, 1.0, 1.0), ic="sedov_offset", seed=6, nsteps=30,
+        npassive=1, description="off-centre blast with reflective walls carrying a passive tracer, HLL"))
+_reg(_w(name="amr_small_passive", mhd=False, solver="hllc", limiter="woodward", gamma_expr="5.0/3.0",
+        gamma=5.0 / 3.0, cfl=0.4, visc_eta=5.0e-3, eight_wave=False, bc=FREE6,
+        nroot=(1, 1, 1), maxlev=4, base_level=2, phys=(20.0 * PC, 20.0 * PC, 20.0 * PC), ic="snr", seed=8,
+        nsteps=8, cooling=True, refine_rule="snr", units="phys", **_PHYS_UNITS, npassive=1,
+        description="small SNR on 3 AMR levels with radiative cooling carrying a passive tracer"))
 # the first-order scheme (SOLVER_HLL1: one Godunov(1) stage, hydro_solver.f90:89-90)
 _reg(_w(name="sedov_hll1", mhd=False, solver="hll1", limiter="minmod", gamma_expr="5.0/3.0",
         gamma=5.0 / 3.0, cfl=0.4, visc_eta=5.0e-3, eight_wave=False, bc=FREE6,
@@ -538,6 +557,8 @@ def _prim_to_cons(w: Workload, rho, vx, vy, vz, p, bx=None, by=None, bz=None):
         u[5] = bx
         u[6] = by
         u[7] = bz
+    if w.npassive:
+        u[nv - w.npassive:] = 0.0
     return u
 
 
@@ -554,6 +575,23 @@ def _gidx(m: Mesh, bid: int):
 
 
 def block_ic(w: Workload, m: Mesh, bid: int) -> np.ndarray:
+    u = _block_ic_flow(w, m, bid)
+    if w.npassive:
+        # tracers as densities: rho on the left of the domain's mid-plane in
+        # x and 0.1 rho on the right; then rho (0.5 + 0.25 sin(2 pi y / L))
+        X, Y, Z = m.cell_centres(bid)
+        L = w.phys[0] / w.l_sc
+        base = w.nv - w.npassive
+        for q in range(w.npassive):
+            if q == 0:
+                f = np.where(X < 0.5 * L, 1.0, 0.1)
+            else:
+                f = 0.5 + 0.25 * np.sin(2.0 * math.pi * Y / L)
+            u[base + q] = u[0] * f
+    return u
+
+
+def _block_ic_flow(w: Workload, m: Mesh, bid: int) -> np.ndarray:
     X, Y, Z = m.cell_centres(bid)
     l, g = _gidx(m, bid)
     zeros = np.zeros_like(X)
