@@ -197,7 +197,8 @@ def main(argv):
     return 0
 
 
-DROPIN_WORKLOADS = ["sedov", "ot64", "amr_small", "blast_reflect", "plane_wind", "sedov_gravity"]
+DROPIN_WORKLOADS = ["sedov", "ot64", "amr_small", "blast_reflect", "plane_wind", "sedov_gravity", "sedov_passive",
+                    "amr_small_passive"]
 # (workload, MPI ranks): 16 ranks = the host cores one GPU of the box comes with
 BASELINE_BUILDS = [("ot64", 16)]
 # workloads whose fixtures include the reference's mesh adaptation

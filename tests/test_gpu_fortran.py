@@ -19,7 +19,8 @@ from walicxe3d_amd import workloads as W
 pytestmark = pytest.mark.gpu
 sys.path.insert(0, GOLDEN)
 
-CASES = ["sedov", "ot64", "amr_small", "blast_reflect", "plane_wind", "sedov_gravity"]
+CASES = ["sedov", "ot64", "amr_small", "blast_reflect", "plane_wind", "sedov_gravity", "sedov_passive",
+         "amr_small_passive"]
 
 
 @pytest.mark.parametrize("name", CASES)

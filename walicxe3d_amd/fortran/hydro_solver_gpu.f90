@@ -25,9 +25,10 @@
 ! arrays as before.
 ! Supported: SOLVER_HLL1/HLL/HLLC/HLLE/HLLD (the fused forms: HLL/HLLC for
 ! hydro, HLLE/HLLD for MHD), COOL_NONE/COOL_TABLE, one MPI rank per GPU.
-! Passive scalars are not carried on the device: a run whose passive scalars
-! are not all zero stops with a message (zero passives stay zero under the
-! reference's update, so they are left as they are).  userBoundary effects
+! Passive scalars: the plain FP64 form carries one or two for hydro (without
+! wind sources) with the flow variables; otherwise a run whose passive
+! scalars are not all zero stops with a message (zero passives stay zero
+! under the reference's update, so they are left as they are).  userBoundary effects
 ! other than periodic faces (handled by the library) must be registered
 ! through hydro_gpu_set_winds (spherical sources) or hydro_gpu_set_plane_wind.
 ! With user_source_terms on, get_user_source_terms runs on the device only as
@@ -56,6 +57,7 @@ module hydro_solver
   integer, save :: nwind = 0
   integer(c_int), save :: pw_plane = 0         ! plane wind (hydro_gpu_set_plane_wind)
   real(c_double), save :: pw_state(4) = 0.0d0  ! rho, vel, temp, mu
+  integer, save :: nvg = 0                     ! variables on the device: neqhydro+neqmhd (+ passives)
   integer(c_int), save :: grav_on = 0          ! uniform acceleration (hydro_gpu_set_gravity)
   real(c_double), save :: grav_g(3) = 0.0d0    ! gx, gy, gz (code units)
 
@@ -143,15 +145,24 @@ contains
     end select
   end subroutine choose_form
 
-  !> Passive scalars are not carried on the device.  All-zero passives stay
-  !! zero under the reference's update, so they are left untouched; any other
+  !> Whether the device carries the passive scalars: the plain FP64 form,
+  !! hydro, one or two passives, no wind sources (whose state leaves the
+  !! passives unset).
+  logical function carry_passives()
+    use parameters
+    carry_passives = npassive >= 1 .and. npassive <= 2 .and. .not. mhd .and. &
+                     gpu_form == WX_FORM_PLAIN_FP64 .and. nwind == 0 .and. pw_plane == 0
+  end function carry_passives
+
+  !> Passive scalars the device does not carry: all-zero passives stay zero
+  !! under the reference's update, so they are left untouched; any other
   !! value stops the run.
   subroutine check_passives()
     use parameters
     use globals
     use clean_quit, only : clean_abort
     integer :: nb
-    if (npassive <= 0) return
+    if (npassive <= 0 .or. carry_passives()) return
     do nb = 1, nbMaxProc
       if (localBlocks(nb) /= -1) then
         if (any(U(nb, firstpas:neqtot, :, :, :) /= 0.0)) then
@@ -185,7 +196,9 @@ contains
     integer(c_int) :: rc
 
     if (.not. c_associated(ctx)) then
-      cfg%nv = neqhydro + neqmhd
+      nvg = neqhydro + neqmhd
+      if (carry_passives()) nvg = neqtot
+      cfg%nv = nvg
       cfg%mhd = merge(1, 0, mhd)
       cfg%solver = solver_type
       cfg%limiter = limiter_type
@@ -232,7 +245,7 @@ contains
                                                       pw_state(4)), "wx_set_plane_wind")
       if (grav_on /= 0) call check(wx_set_uniform_gravity(ctx, grav_on, grav_g(1), grav_g(2), grav_g(3)), &
                                    "wx_set_uniform_gravity")
-      allocate(ubuf(nbMaxProc, neqhydro + neqmhd, nxmin:nxmax, nymin:nymax, nzmin:nzmax))
+      allocate(ubuf(nbMaxProc, nvg, nxmin:nxmax, nymin:nymax, nzmin:nzmax))
       allocate(lbuf(nbMaxProc))
     end if
 
@@ -269,8 +282,6 @@ contains
   subroutine to_device()
     use parameters
     use globals
-    integer :: nvg
-    nvg = neqhydro + neqmhd
     lbuf(:) = localBlocks(1:nbMaxProc)
     ubuf(:,:,:,:,:) = U(:, 1:nvg, :, :, :)
     call check(wx_upload_state(ctx, c_loc(ubuf), WX_LAYOUT_FORTRAN, int(nbMaxProc,c_int), c_loc(lbuf)), &
@@ -284,8 +295,6 @@ contains
   subroutine to_device_u()
     use parameters
     use globals
-    integer :: nvg
-    nvg = neqhydro + neqmhd
     lbuf(:) = localBlocks(1:nbMaxProc)
     ubuf(:,:,:,:,:) = U(:, 1:nvg, :, :, :)
     call check(wx_upload_state(ctx, c_loc(ubuf), WX_LAYOUT_FORTRAN, int(nbMaxProc,c_int), c_loc(lbuf)), &
@@ -296,8 +305,6 @@ contains
   subroutine to_host()
     use parameters
     use globals
-    integer :: nvg
-    nvg = neqhydro + neqmhd
     ubuf(:,:,:,:,:) = U(:, 1:nvg, :, :, :)
     call check(wx_download_state(ctx, c_loc(ubuf), c_null_ptr, WX_LAYOUT_FORTRAN, int(nbMaxProc,c_int), &
                c_loc(lbuf)), "wx_download_state")
@@ -324,8 +331,8 @@ contains
       write(logu,'(a)') "user_source_terms: get_user_source_terms does not run on the GPU (see hydro_gpu_set_gravity)"
       call clean_abort (ERROR_GENERIC)
     end if
-    call check_passives()
     call choose_form()
+    call check_passives()
     call ensure_context()
     if (gpu_form /= WX_FORM_PLAIN_FP64) then
       ! one device step: dt (CFL minimum, start-up ramp, output clamp), both
