@@ -53,7 +53,8 @@ SOLVER = {"hll1": "SOLVER_HLL1", "hll": "SOLVER_HLL", "hllc": "SOLVER_HLLC",
 LIMITER = {"noavg": "LIMITER_NO_AVERAGE", "none": "LIMITER_NONE", "minmod": "LIMITER_MINMOD",
            "vanleer": "LIMITER_VANLEER", "albada": "LIMITER_ALBADA", "umist": "LIMITER_UMIST",
            "woodward": "LIMITER_WOODWARD", "superbee": "LIMITER_SUPERBEE"}
-EOS = {"adiabatic": "EOS_ADIABATIC", "single": "EOS_SINGLE_SPECIE", "twotemp": "EOS_TWOTEMP"}
+EOS = {"adiabatic": "EOS_ADIABATIC", "single": "EOS_SINGLE_SPECIE", "twotemp": "EOS_TWOTEMP",
+       "hrate": "EOS_H_RATE"}
 
 
 def flit(x):

@@ -436,8 +436,11 @@ void wxo_riemann(const wxo_config *c, int solver, const double *pl, const double
 /* cooling.f90 (COOL_TABLE) and calcTemp                               */
 /* ------------------------------------------------------------------ */
 
-/* calcTemp (source/hydro_core.f90:231-270), EOS_ADIABATIC/SINGLE/TWOTEMP */
+/* calcTemp (source/hydro_core.f90:231-270), EOS_ADIABATIC/SINGLE/TWOTEMP,
+   and EOS_H_RATE (:258-264: the neutral hydrogen density in the first
+   passive scalar; hydro with passives) */
 double wxo_calc_temp(const wxo_config *c, const double *p) {
+  if (c->eos == 3) return p[4] / (2.0 * p[0] / c->mu0 - p[5]) * c->p_sc / KB;
   double t = p[4] / p[0] * (c->mu0 * AMU * c->p_sc / c->d_sc / KB);
   if (c->eos == 2 && t > c->ion_thres) t = p[4] / p[0] * (c->mui * AMU * c->p_sc / c->d_sc / KB);
   return t;

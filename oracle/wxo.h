@@ -24,7 +24,7 @@ typedef struct {
   int limiter;       /* LIMITER_* (-1..6) */
   int eight_wave;    /* Powell source term */
   int cooling;       /* 0 = none, 1 = COOL_TABLE */
-  int eos;           /* EOS_ADIABATIC=0, SINGLE_SPECIE=1, TWOTEMP=2 */
+  int eos;           /* EOS_ADIABATIC=0, SINGLE_SPECIE=1, TWOTEMP=2, H_RATE=3 */
   int bc[6];         /* BC_REFLECTIVE=1, FREE_FLOW=2, PERIODIC=3 per face */
   double gamma, cv, cfl, visc_eta;
   double cooling_limit, mu0, mui, ion_thres;

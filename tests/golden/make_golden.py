@@ -59,6 +59,7 @@ TRAJ = {
     "sedov_passive": (20, [5, 20]),
     "blast_reflect_passive": (30, [10, 30]),
     "amr_small_passive": (8, [2, 8]),
+    "amr_small_hrate": (8, [2, 8]),
 }
 NSAMPLE = 4096
 # workloads run with admesh after every step: (steps, dumped steps)

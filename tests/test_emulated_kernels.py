@@ -253,7 +253,7 @@ def test_emulated_passive_scalars_reproduce_the_oracle(emu, tmp_path, oracle):
     blast): bit-exact to the oracle, tracers included."""
     import dataclasses
     cases = [(W.get("sedov_passive"), 2), (dataclasses.replace(W.get("sedov_passive"), npassive=2), 2),
-             (W.get("blast_reflect_passive"), 2), (W.get("amr_small_passive"), 1)]
+             (W.get("blast_reflect_passive"), 2), (W.get("amr_small_passive"), 1), (W.get("amr_small_hrate"), 2)]
     for n, (w, steps) in enumerate(cases):
         p = str(tmp_path / ("passive%d.bin" % n))
         blocks, U0 = _write_input(p, w, hip.PLAIN_FP64)

@@ -38,7 +38,8 @@ def gpu(w, form, bids):
 PLAIN_CASES = [("sedov", 10), ("briowu", 30), ("ot64", 8), ("blast_reflect", 30), ("mhd_hlle", 20),
                ("amr_small", 8), ("snr_amr4", 3), ("winds4", 3), ("ot64_amr", 8), ("plane_wind", 5),
                ("sedov_hll1", 20), ("amr_small_twotemp", 8), ("sedov_gravity", 20), ("ot64_gravity", 8),
-               ("sedov_passive", 20), ("blast_reflect_passive", 30), ("amr_small_passive", 8)]
+               ("sedov_passive", 20), ("blast_reflect_passive", 30), ("amr_small_passive", 8),
+               ("amr_small_hrate", 8)]
 
 
 @pytest.mark.parametrize("name,nsteps", PLAIN_CASES)
@@ -137,7 +138,7 @@ def test_other_limiters_match_oracle(oracle, name, lim):
 @pytest.mark.parametrize("name", ["sedov", "briowu", "ot64", "blast_reflect", "mhd_hlle", "amr_small",
                                   "snr_amr4", "winds4", "ot64_amr", "plane_wind", "sedov_hll1",
                                   "amr_small_twotemp", "sedov_gravity", "ot64_gravity", "sedov_passive",
-                                  "blast_reflect_passive", "amr_small_passive"])
+                                  "blast_reflect_passive", "amr_small_passive", "amr_small_hrate"])
 def test_plain_fp64_matches_reference_fixture(name):
     from walicxe3d_amd import hip
     fx = load_golden("traj_%s.npz" % name)

@@ -76,7 +76,7 @@ def _first_dump(fx):
 # GPU tests check the last dump)
 TRAJ = ["sedov", "briowu", "ot64", "blast_reflect", "mhd_hlle", "amr_small", "snr_amr4", "winds4", "ot64_amr",
         "plane_wind", "sedov_hll1", "amr_small_twotemp", "sedov_gravity", "ot64_gravity",
-        "sedov_passive", "blast_reflect_passive", "amr_small_passive"]
+        "sedov_passive", "blast_reflect_passive", "amr_small_passive", "amr_small_hrate"]
 
 
 @pytest.mark.parametrize("name", TRAJ)
@@ -159,3 +159,19 @@ def test_a_tracer_equal_to_the_density_stays_the_density(oracle, name, steps, to
     d = np.abs(u[:, 5] - u[:, 0]).max() / np.abs(u[:, 0]).max()
     assert d <= tol, d
     assert np.array_equal(p[:, 5], u[:, 5])
+
+
+def test_hrate_fixture_tells_the_eos_apart(oracle):
+    """The EOS_H_RATE fixture pins calcTemp's neutral-hydrogen branch: the
+    same run with the adiabatic EOS misses it."""
+    import dataclasses
+    fx = load_golden("traj_amr_small_hrate.npz")
+    w = dataclasses.replace(W.get("amr_small_hrate"), eos="adiabatic")
+    bids = fx["bids"]
+    sim = oracle.Sim(w, bids)
+    sim.set_state(ic_in_order(w, bids))
+    for _ in range(_first_dump(fx)):
+        sim.step()
+    u, p = sim.get_state()
+    dev_sum, dev_s = compare_dump(w, fx, _first_dump(fx), u, p, bids)
+    assert dev_sum > 0.0 or dev_s > 0.0

@@ -853,9 +853,13 @@ int wx_init(const wx_config *cfg, void **out) {
     return fail(WX_EINVAL, "the fused form with nv = 8 runs HLLE or HLLD");
   if (cfg->form != WX_FORM_PLAIN_FP64 && cfg->nv == 8 && !cfg->mhd)
     return fail(WX_EINVAL, "the fused form with nv = 8 needs mhd = 1");
-  // calcTemp (source/hydro_core.f90:231-270): EOS_H_RATE (3), which takes the
-  // neutral fraction from the first passive scalar, is not implemented
-  if (cfg->eos < 0 || cfg->eos > 2) return fail(WX_EINVAL, "eos must be EOS_ADIABATIC, EOS_SINGLE_SPECIE or EOS_TWOTEMP");
+  // calcTemp (source/hydro_core.f90:231-270): EOS_H_RATE (3) takes the
+  // neutral hydrogen density from the first passive scalar (plain form)
+  if (cfg->eos < 0 || cfg->eos > 3)
+    return fail(WX_EINVAL, "eos must be EOS_ADIABATIC, EOS_SINGLE_SPECIE, EOS_TWOTEMP or EOS_H_RATE");
+  if (cfg->eos == 3 && cfg->nv != 6 && cfg->nv != 7)
+    return fail(WX_EINVAL, "EOS_H_RATE reads the neutral hydrogen density from the first passive scalar "
+                           "(hydro, nv = 6 or 7, plain FP64 form)");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(WX_EHIP, "no HIP device");
   Ctx *c = new Ctx();
@@ -1120,7 +1124,7 @@ int wx_cooling(void *ctx, double dt) {
   if (rc != WX_OK) return rc;
   if (is_fused(c)) return fail(WX_EINVAL, "plain-form entry point");
   if (!c->cfg.cooling) return WX_OK;
-  plain_cooling(c->dc, c->dU, c->dP, dt, c->st);
+  plain_cooling(c->dc, c->dU, c->dP, dt, c->cfg.mu0, c->st);
   return launch_check("cooling");
 }
 
@@ -1150,7 +1154,7 @@ int wx_step(void *ctx, int *it, double *dt, double *time, int *nextout, double d
   long bad = 0;
   rc = update_prims(c, &bad);
   if (rc != WX_OK) return rc;
-  if (c->cfg.cooling) plain_cooling(c->dc, c->dU, c->dP, dtl, c->st);
+  if (c->cfg.cooling) plain_cooling(c->dc, c->dU, c->dP, dtl, c->cfg.mu0, c->st);
   if (dump) *nextout += 1;
   rc = launch_check("step");
   return rc != WX_OK ? rc : (int)bad;

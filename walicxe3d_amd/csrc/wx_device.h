@@ -255,9 +255,12 @@ __device__ inline double calc_temp(const DevCfg &c, double rho, double p) {
 }
 
 // One cell; prim holds (rho, u, v, w, p).  Returns true when the cell
-// changed, with new p in prim[4] and new total energy in *e.
-__device__ inline bool cool_cell(const DevCfg &c, double dt, double *prim, double *e) {
-  double temp = calc_temp(c, prim[0], prim[4]);
+// changed, with new p in prim[4] and new total energy in *e.  have_t: the
+// caller's temperature t_in replaces calcTemp's (EOS_H_RATE, which needs a
+// passive scalar; plain form).
+__device__ inline bool cool_cell(const DevCfg &c, double dt, double *prim, double *e, bool have_t = false,
+                                 double t_in = 0.0) {
+  double temp = have_t ? t_in : calc_temp(c, prim[0], prim[4]);
   double logT = log10(temp);
   if (!(logT >= c.tabT[0])) return false;
   double ETH = c.pd.cv * prim[4];

@@ -15,7 +15,7 @@ void plain_upwind(const DevCfg &c, const UserSource &us, const double *U, const 
                   double *UP, double dtp, hipStream_t st);
 void plain_viscous(const DevCfg &c, const double *UP, double *U, hipStream_t st);
 void plain_timestep(const DevCfg &c, const double *P, unsigned long long *dtbits, hipStream_t st);
-void plain_cooling(const DevCfg &c, double *U, double *P, double dt, hipStream_t st);
+void plain_cooling(const DevCfg &c, double *U, double *P, double dt, double mu0, hipStream_t st);
 // per-block largest relative pressure / density gradient (markByPhysical)
 void plain_mark(const DevCfg &c, const double *P, double *maxgrad, hipStream_t st);
 // the same on the fused state ([blk][nv][16^3]): recorded ghost layer

@@ -375,7 +375,7 @@ __global__ void __launch_bounds__(256) k_timestep(DevCfg c, const double *P, uns
 // ---------------------------------------------------------------------------
 // cooling (source/cooling.f90:37-118, apply_cooling :285-371).  grid (16, nblk)
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_cooling(DevCfg c, double *U, double *P, double dt) {
+__global__ void __launch_bounds__(256) k_cooling(DevCfg c, double *U, double *P, double dt, double mu0) {
   int b = blockIdx.y;
   int t = blockIdx.x * 256 + threadIdx.x;
   int i = t % NC + 1, j = (t / NC) % NC + 1, k = t / (NC * NC) + 1;
@@ -383,8 +383,13 @@ __global__ void __launch_bounds__(256) k_cooling(DevCfg c, double *U, double *P,
   double *Pb = P + (size_t)b * nv * NCELL;
   double pr[5];
   for (int v = 0; v < 5; v++) pr[v] = Pb[v * NCELL + q];
+  // EOS_H_RATE (calcTemp, hydro_core.f90:258-264): P = (n_HI + n_HII) k T
+  // with the neutral hydrogen density in the first passive scalar
+  const bool hrate = c.eos == 3;
+  double th = 0.0;
+  if (hrate) th = pr[4] / (2.0 * pr[0] / mu0 - Pb[5 * NCELL + q]) * c.p_sc / KB;
   double e;
-  if (cool_cell(c, dt, pr, &e)) {
+  if (cool_cell(c, dt, pr, &e, hrate, th)) {
     Pb[4 * NCELL + q] = pr[4];
     U[((size_t)b * nv + 4) * NCELL + q] = e;
   }
@@ -557,8 +562,8 @@ void plain_viscous(const DevCfg &c, const double *UP, double *U, hipStream_t st)
 void plain_timestep(const DevCfg &c, const double *P, unsigned long long *dtbits, hipStream_t st) {
   hipLaunchKernelGGL(k_timestep, dim3(16, c.nblk), dim3(256), 0, st, c, P, dtbits);
 }
-void plain_cooling(const DevCfg &c, double *U, double *P, double dt, hipStream_t st) {
-  hipLaunchKernelGGL(k_cooling, dim3(16, c.nblk), dim3(256), 0, st, c, U, P, dt);
+void plain_cooling(const DevCfg &c, double *U, double *P, double dt, double mu0, hipStream_t st) {
+  hipLaunchKernelGGL(k_cooling, dim3(16, c.nblk), dim3(256), 0, st, c, U, P, dt, mu0);
 }
 void plain_mark(const DevCfg &c, const double *P, double *maxgrad, hipStream_t st) {
   hipLaunchKernelGGL(k_mark, dim3(c.nblk), dim3(256), 0, st, c, P, maxgrad);

@@ -37,7 +37,7 @@ SOLVERS = {"hll1": 2, "hll": 3, "hllc": 4, "hlle": 5, "hlld": 6}
 LIMITERS = {"noavg": -1, "none": 0, "minmod": 1, "vanleer": 2, "albada": 3,
             "umist": 4, "woodward": 5, "superbee": 6}
 BCS = {"reflective": 1, "free": 2, "periodic": 3}
-EOSS = {"adiabatic": 0, "single": 1, "twotemp": 2}
+EOSS = {"adiabatic": 0, "single": 1, "twotemp": 2, "hrate": 3}
 
 
 # ----------------------------------------------------------------------------
@@ -304,6 +304,13 @@ _reg(_w(name="amr_small_passive", mhd=False, solver="hllc", limiter="woodward", 
         nroot=(1, 1, 1), maxlev=4, base_level=2, phys=(20.0 * PC, 20.0 * PC, 20.0 * PC), ic="snr", seed=8,
         nsteps=8, cooling=True, refine_rule="snr", units="phys", **_PHYS_UNITS, npassive=1,
         description="small SNR on 3 AMR levels with radiative cooling carrying a passive tracer"))
+# EOS_H_RATE (calcTemp, hydro_core.f90:258-264): the tracer is the neutral
+# hydrogen density of the cooling temperature
+_reg(_w(name="amr_small_hrate", mhd=False, solver="hllc", limiter="woodward", gamma_expr="5.0/3.0",
+        gamma=5.0 / 3.0, cfl=0.4, visc_eta=5.0e-3, eight_wave=False, bc=FREE6,
+        nroot=(1, 1, 1), maxlev=4, base_level=2, phys=(20.0 * PC, 20.0 * PC, 20.0 * PC), ic="snr", seed=8,
+        nsteps=8, cooling=True, refine_rule="snr", eos="hrate", units="phys", **_PHYS_UNITS, npassive=1,
+        description="small SNR on 3 AMR levels with cooling, EOS_H_RATE (neutral hydrogen as the tracer)"))
 # the first-order scheme (SOLVER_HLL1: one Godunov(1) stage, hydro_solver.f90:89-90)
 _reg(_w(name="sedov_hll1", mhd=False, solver="hll1", limiter="minmod", gamma_expr="5.0/3.0",
         gamma=5.0 / 3.0, cfl=0.4, visc_eta=5.0e-3, eight_wave=False, bc=FREE6,
