@@ -68,11 +68,14 @@ def test_rejects_bad_config_before_touching_a_device():
     cfg.solver = 2   # first-order HLL runs only in the plain form
     assert L.wx_init(C.byref(cfg), C.byref(h)) == -1
     assert L.wx_set_mesh(None, None, 1, None, 1, None) == -1
-    # EOS_H_RATE (the neutral fraction from a passive scalar) is not taken
+    # an unknown EOS; EOS_H_RATE without the passive scalar it reads
     cfg = hip.make_config(W.get("amr_small"))
-    cfg.eos = 3
+    cfg.eos = 4
     assert L.wx_init(C.byref(cfg), C.byref(h)) == -1
     assert b"eos" in L.wx_last_error()
+    cfg.eos = 3
+    assert L.wx_init(C.byref(cfg), C.byref(h)) == -1
+    assert b"passive scalar" in L.wx_last_error()
     # the fused form runs HLLE/HLLD (and MHD) for 8 variables only
     cfg = hip.make_config(W.get("ot64"), form=hip.FUSED_FP32)
     cfg.solver = 4
