@@ -276,8 +276,9 @@ int wx_eval_physics_noalloc(const wx_config *cfg, int op, int arg, int n, const 
 /* Checkpoint files of the reference, byte for byte.  writeBin
    (source/output.f90:156-214): the device state to a stream file (int32
    block count; per block int32 bID and U(1:neqtot,1:16,1:16,1:16), variable
-   fastest), neqtot = nv + npassive with the passive columns, which the
-   library does not carry, written as zero. */
+   fastest), neqtot = flow variables + npassive: the passive scalars the
+   context carries (nv = 6, 7), then zero columns for the rest; npassive
+   below the carried count is refused. */
 int wx_write_bin(void *ctx, const char *path, int npassive);
 /* warmstart's read of that file (source/init.f90:582-610) into host arrays:
    bids[cap] and u[cap][nv][16][16][16] (WX_LAYOUT_BLOCKS), passive columns
@@ -294,7 +295,7 @@ int wx_read_state(const char *path, double t_sc, double *time, int *it, int *nex
 /* The VTK part of writeOutput (source/output.f90:38-151) on one rank:
    updatePrims, then write3DVTKBlocks (:273-750: datadir/Blocks000.NNNN.vtk,
    every cell a hexahedron, rho, vel, P, B, T and npassive passive scalars
-   (zero, not carried), FP32 big-endian; master.visit replaced at output 0
+   (zero; a context that carries passive scalars is refused), FP32 big-endian; master.visit replaced at output 0
    and appended to after) and write3DVTKGrid (:755-895: datadir/Grid.NNNN.vtk,
    one hexahedron per block with rank, bID and Hilbert key).  units is the
    reference's units_type: 1 PHYS_UNITS (scaled by l_sc, d_sc, v_sc, p_sc,

@@ -105,8 +105,9 @@ static int run_case(const char *path, int nsteps) {
   if (vtkdir && wx_write_vtk(ctx, vtkdir, nsteps, 1, vtkunits) < 0) { printf("vtk %s\n", wx_last_error()); return 1; }
   // the reference's checkpoint file of the final state (one zero passive
   // column, as the oracle builds of the reference carry one passive scalar)
-  // the writers take contexts without passive scalars (nv 5 or 8)
-  if (cfg.nv == 5 || cfg.nv == 8) { std::string fn = std::string(path) + ".wxbin"; if (wx_write_bin(ctx, fn.c_str(), 1) < 0) { printf("write_bin %s\n", wx_last_error()); return 1; } }
+  // (hydro with passives: the file's passive columns hold the carried ones)
+  { const int np = (cfg.nv == 6 || cfg.nv == 7) ? cfg.nv - 5 : 1;
+    std::string fn = std::string(path) + ".wxbin"; if (wx_write_bin(ctx, fn.c_str(), np) < 0) { printf("write_bin %s\n", wx_last_error()); return 1; } }
   double s = 0; for (double x : Uo) s += x;
   printf("it %d dt %.17g time %.17g sum %.17g\n", it, dt, time, s);
   wx_finalize(ctx);

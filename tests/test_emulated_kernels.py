@@ -265,6 +265,15 @@ def test_emulated_passive_scalars_reproduce_the_oracle(emu, tmp_path, oracle):
         uo, _ = o.get_state()
         u = np.fromfile(p + ".out").reshape(uo.shape)
         assert np.array_equal(u, uo), (w.name, w.npassive, [float(np.abs(u[:, v] - uo[:, v]).max()) for v in range(w.nv)])
+        # the checkpoint file (writeBin layout: per block the bID, then
+        # [cell][neqtot], variable fastest) holds the tracers as its passives
+        raw = open(p + ".wxbin", "rb").read()
+        nb = int(np.frombuffer(raw[:4], dtype="<i4")[0])
+        assert nb == u.shape[0]
+        rec = 4 + 8 * w.nv * 4096
+        for b in (0, nb - 1):
+            data = np.frombuffer(raw[4 + b * rec + 4: 4 + (b + 1) * rec], dtype="<f8").reshape(4096, w.nv)
+            assert np.array_equal(data.T, u[b].reshape(w.nv, 4096))
 
 
 def test_emulated_admesh_reproduces_the_oracle(emu, tmp_path, oracle):

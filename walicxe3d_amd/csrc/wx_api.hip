@@ -1445,20 +1445,26 @@ int wx_hilbert_keys(const int nroot[3], int maxlev, int nblk, const int *bids, i
   return WX_OK;
 }
 
-// The writers take the flow variables of the state and write npassive zero
-// passive columns: a context that carries its own passive scalars is refused.
+// Passive scalars the context carries (nv 6, 7: hydro with one or two).
+static int carried_passives(const Ctx *c) { return (c->cfg.nv == 6 || c->cfg.nv == 7) ? c->cfg.nv - 5 : 0; }
+
+// The VTK writer takes the flow variables and writes npassive zero passive
+// fields: a context that carries its own passive scalars is refused.
 static int writers_take_flow_variables(const Ctx *c) {
-  if (c->cfg.nv == 6 || c->cfg.nv == 7)
-    return fail(WX_EINVAL, "the output writers take contexts without passive scalars");
+  if (carried_passives(c))
+    return fail(WX_EINVAL, "the VTK writer takes contexts without passive scalars");
   return WX_OK;
 }
 
 int wx_write_bin(void *ctx, const char *path, int npassive) {
   Ctx *c = as_ctx(ctx);
   int rc = need_state(c);
-  if (rc == WX_OK) rc = writers_take_flow_variables(c);
   if (rc != WX_OK) return rc;
-  if (!path || npassive < 0) return fail(WX_EINVAL, "bad argument");
+  // the file holds neqtot = flow variables + npassive columns: the passives
+  // the context carries, then zero columns for the rest
+  const int carried = carried_passives(c);
+  if (!path || npassive < carried) return fail(WX_EINVAL, "bad argument (npassive below the passives carried)");
+  npassive -= carried;
   std::vector<double> u((size_t)c->nblk * c->cfg.nv * NINT);
   rc = wx_download_state(ctx, u.data(), nullptr, WX_LAYOUT_BLOCKS, 0, nullptr);
   if (rc < 0) return rc;
