@@ -295,7 +295,8 @@ int wx_read_state(const char *path, double t_sc, double *time, int *it, int *nex
 /* The VTK part of writeOutput (source/output.f90:38-151) on one rank:
    updatePrims, then write3DVTKBlocks (:273-750: datadir/Blocks000.NNNN.vtk,
    every cell a hexahedron, rho, vel, P, B, T and npassive passive scalars
-   (zero; a context that carries passive scalars is refused), FP32 big-endian; master.visit replaced at output 0
+   (those the context carries, zero for the rest; npassive below the
+   carried count is refused), FP32 big-endian; master.visit replaced at output 0
    and appended to after) and write3DVTKGrid (:755-895: datadir/Grid.NNNN.vtk,
    one hexahedron per block with rank, bID and Hilbert key).  units is the
    reference's units_type: 1 PHYS_UNITS (scaled by l_sc, d_sc, v_sc, p_sc,

@@ -60,8 +60,9 @@ static int run_case(const char *path, int nsteps) {
     rc = wx_set_tuning(ctx, 4, atoi(getenv("EMU_NSLAB"))); if (rc < 0) { printf("slabs %d %s\n", rc, wx_last_error()); return 1; }
   }
   const char *vtkdir = getenv("EMU_VTK_DIR");
+  const int npv = (cfg.nv == 6 || cfg.nv == 7) ? cfg.nv - 5 : 1;   // passive fields: the carried ones, or one zero
   const int vtkunits = getenv("EMU_VTK_UNITS") ? atoi(getenv("EMU_VTK_UNITS")) : 2;
-  if (vtkdir && wx_write_vtk(ctx, vtkdir, 0, 1, vtkunits) < 0) { printf("vtk %s\n", wx_last_error()); return 1; }
+  if (vtkdir && wx_write_vtk(ctx, vtkdir, 0, npv, vtkunits) < 0) { printf("vtk %s\n", wx_last_error()); return 1; }
   int it = 0, nextout = 1; double dt = 0, time = 0;
   if (nb2 == -1) {
     std::vector<int> nb(8 * 4096);
@@ -102,7 +103,7 @@ static int run_case(const char *path, int nsteps) {
   std::vector<double> Uo(U.size());
   wx_download_state(ctx, Uo.data(), nullptr, WX_LAYOUT_BLOCKS, 0, nullptr);
   { std::string fn = std::string(path) + ".out"; FILE *fo = fopen(fn.c_str(), "wb"); fwrite(Uo.data(), 8, Uo.size(), fo); fclose(fo); }
-  if (vtkdir && wx_write_vtk(ctx, vtkdir, nsteps, 1, vtkunits) < 0) { printf("vtk %s\n", wx_last_error()); return 1; }
+  if (vtkdir && wx_write_vtk(ctx, vtkdir, nsteps, npv, vtkunits) < 0) { printf("vtk %s\n", wx_last_error()); return 1; }
   // the reference's checkpoint file of the final state (one zero passive
   // column, as the oracle builds of the reference carry one passive scalar)
   // (hydro with passives: the file's passive columns hold the carried ones)

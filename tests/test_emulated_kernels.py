@@ -257,7 +257,9 @@ def test_emulated_passive_scalars_reproduce_the_oracle(emu, tmp_path, oracle):
     for n, (w, steps) in enumerate(cases):
         p = str(tmp_path / ("passive%d.bin" % n))
         blocks, U0 = _write_input(p, w, hip.PLAIN_FP64)
-        _run(emu, [p, str(steps)], str(tmp_path))
+        vtkdir = tmp_path / ("vtk%d" % n)
+        vtkdir.mkdir()
+        _run(emu, [p, str(steps)], str(tmp_path), EMU_VTK_DIR=str(vtkdir) + "/", EMU_VTK_UNITS="2")
         o = oracle.Sim(w, blocks)
         o.set_state(U0)
         for _ in range(steps):
@@ -274,6 +276,14 @@ def test_emulated_passive_scalars_reproduce_the_oracle(emu, tmp_path, oracle):
         for b in (0, nb - 1):
             data = np.frombuffer(raw[4 + b * rec + 4: 4 + (b + 1) * rec], dtype="<f8").reshape(4096, w.nv)
             assert np.array_equal(data.T, u[b].reshape(w.nv, 4096))
+        # the VTK blocks file's passive1 field (write3DVTKBlocks,
+        # output.f90:711-737) holds the tracer as FP32, big-endian
+        if w.npassive == 1:
+            vtk = (vtkdir / ("Blocks000.%04d.vtk" % steps)).read_bytes()
+            key = b"SCALARS passive1 float 1\nLOOKUP_TABLE default\n"
+            at = vtk.index(key) + len(key)
+            field = np.frombuffer(vtk[at: at + 4 * u.shape[0] * 4096], dtype=">f4")
+            assert np.array_equal(field, u[:, 5].reshape(-1).astype(np.float32))
 
 
 def test_emulated_admesh_reproduces_the_oracle(emu, tmp_path, oracle):

@@ -1448,14 +1448,6 @@ int wx_hilbert_keys(const int nroot[3], int maxlev, int nblk, const int *bids, i
 // Passive scalars the context carries (nv 6, 7: hydro with one or two).
 static int carried_passives(const Ctx *c) { return (c->cfg.nv == 6 || c->cfg.nv == 7) ? c->cfg.nv - 5 : 0; }
 
-// The VTK writer takes the flow variables and writes npassive zero passive
-// fields: a context that carries its own passive scalars is refused.
-static int writers_take_flow_variables(const Ctx *c) {
-  if (carried_passives(c))
-    return fail(WX_EINVAL, "the VTK writer takes contexts without passive scalars");
-  return WX_OK;
-}
-
 int wx_write_bin(void *ctx, const char *path, int npassive) {
   Ctx *c = as_ctx(ctx);
   int rc = need_state(c);
@@ -1503,9 +1495,8 @@ int wx_read_state(const char *path, double t_sc, double *time, int *it, int *nex
 int wx_write_vtk(void *ctx, const char *datadir, int noutput, int npassive, int units) {
   Ctx *c = as_ctx(ctx);
   int rc = need_state(c);
-  if (rc == WX_OK) rc = writers_take_flow_variables(c);
   if (rc != WX_OK) return rc;
-  if (!datadir || npassive < 0 || noutput < 0 || noutput > 9999 || (units != 1 && units != 2))
+  if (!datadir || npassive < carried_passives(c) || noutput < 0 || noutput > 9999 || (units != 1 && units != 2))
     return fail(WX_EINVAL, "bad argument");
   const int nv = c->cfg.nv;
   std::vector<double> u((size_t)c->nblk * nv * NINT), prim(u.size());
@@ -1517,6 +1508,7 @@ int wx_write_vtk(void *ctx, const char *datadir, int noutput, int npassive, int 
       double uu[MAXV], pp[MAXV];
       for (int v = 0; v < nv; v++) uu[v] = u[((size_t)b * nv + v) * NINT + q];
       flow2prim(c->dc.pd, uu, pp);
+      for (int v = c->dc.pd.nv; v < nv; v++) pp[v] = uu[v];   // passive scalars
       for (int v = 0; v < nv; v++) prim[((size_t)b * nv + v) * NINT + q] = pp[v];
     }
   std::vector<int> levels(c->nblk), coords(3 * (size_t)c->nblk);
@@ -1544,6 +1536,9 @@ int wx_write_vtk(void *ctx, const char *datadir, int noutput, int npassive, int 
   m.temp_mu0 = c->dc.temp_mu0;
   m.temp_mui = c->dc.temp_mui;
   m.ion_thres = c->cfg.ion_thres;
+  m.mu0 = c->cfg.mu0;
+  m.kb = KB;
+  m.ncarried = carried_passives(c);
   std::string dir(datadir);
   if (dir.empty() || dir.back() != '/') dir += '/';
   char name[64];

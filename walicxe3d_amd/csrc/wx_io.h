@@ -36,11 +36,13 @@ struct VtkInfo {
   int nroot[3] = {1, 1, 1};
   double l_sc = 1, d_sc = 1, v_sc = 1, p_sc = 1, b_sc = 1;
   double temp_mu0 = 1, temp_mui = 1, ion_thres = 0;   // calcTemp
+  double mu0 = 1, kb = 1;       // calcTemp, EOS_H_RATE: mu0 and Boltzmann's constant (cgs)
+  int ncarried = 0;             // passive scalars prim holds (after the flow variables)
 };
 
 // write3DVTKBlocks (output.f90:273-750): every cell a hexahedron of its own
-// 8 points; rho, vel, P, B (8 variables), T and the passive scalars (zero:
-// not carried) as FP32, big-endian; prim [blk][nv][16^3].
+// 8 points; rho, vel, P, B (8 variables), T and the passive scalars (those
+// prim carries, zero for the rest) as FP32, big-endian; prim [blk][nv][16^3].
 int vtk_blocks_write(const char *path, const VtkInfo &m, const double *prim, std::string &err);
 // write3DVTKGrid (output.f90:755-895): one hexahedron per block with its
 // rank (0), bID and Hilbert key.

@@ -194,8 +194,10 @@ void ref_corner(const VtkInfo &m, int b, double xb[3]) {
   for (int a = 0; a < 3; a++) xb[a] = (double)((m.coords[3 * b + a] - 1) * NC) * m.dxyz[a * m.maxlev + l - 1];
 }
 
-// calcTemp (hydro_core.f90:231-270), EOS_ADIABATIC / SINGLE_SPECIE / TWOTEMP
-double calc_temp_host(const VtkInfo &m, double rho, double p) {
+// calcTemp (hydro_core.f90:231-270), EOS_ADIABATIC / SINGLE_SPECIE / TWOTEMP,
+// and EOS_H_RATE (:258-264) with the neutral hydrogen density nhi
+double calc_temp_host(const VtkInfo &m, double rho, double p, double nhi) {
+  if (m.eos == 3) return p / (2.0 * rho / m.mu0 - nhi) * m.p_sc / m.kb;
   double t = p / rho * m.temp_mu0;
   if (m.eos == 2 && t > m.ion_thres) t = p / rho * m.temp_mui;
   return t;
@@ -291,14 +293,20 @@ int vtk_blocks_write(const char *path, const VtkInfo &m, const double *prim, std
   w.text("SCALARS T float 1");
   w.text("LOOKUP_TABLE default");
   for (int b = 0; b < m.nblk; b++)
-    for (int q = 0; q < NINT; q++) w.f32((float)calc_temp_host(m, P(b, 0, q), P(b, 4, q)));
+    for (int q = 0; q < NINT; q++)
+      w.f32((float)calc_temp_host(m, P(b, 0, q), P(b, 4, q), m.ncarried ? P(b, 5, q) : 0.0));
   w.lf();
   for (int pas = 1; pas <= m.npassive; pas++) {
     char line[64];
     std::snprintf(line, sizeof line, "SCALARS passive%d float 1", pas);
     w.text(line);
     w.text("LOOKUP_TABLE default");
-    for (long c = 0; c < ncells; c++) w.f32(0.0f);
+    if (pas <= m.ncarried) {
+      for (int b = 0; b < m.nblk; b++)
+        for (int q = 0; q < NINT; q++) w.f32((float)P(b, 4 + pas, q));
+    } else {
+      for (long c = 0; c < ncells; c++) w.f32(0.0f);
+    }
     w.lf();
   }
   if (!w.close()) {
